@@ -1,0 +1,110 @@
+"""Write the reference golden fixtures under tests/golden/ (run in the build container).
+
+Each case compiles the reference Fortran (oracle/refbuild.py: symlinks to
+/root/reference/src, a generated cparam.local, amdflang -O3 -ffp-contract=off, MPICH),
+injects a seeded psi0 through the restart file, runs it, and stores psi0 (or its
+generator parameters) and the reference output.  Only these data files are committed;
+the reference itself never travels.
+
+    python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
+import refbuild  # noqa: E402
+from trapbec_amd import initial  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def small_ic(shape, box, seed, scale=True):
+    """TF (mu = 10, V = r^2/2 in physical units, g = 500) times a z-line vortex plus
+    1e-3 noise, in reference units (psi * sqrt(2g))."""
+    psi, _ = initial.tf_profile(shape, box, (1.0, 1.0, 1.0), 500.0, mu=10.0)
+    psi = psi * initial.vortex_line_factor(shape, box, 0.3, -0.2, 1, 1 / math.sqrt(10))
+    psi = psi + initial.noise(shape, seed, 1e-3)
+    return psi * math.sqrt(1000.0) if scale else psi
+
+
+def case_small(name, prec, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), steps=20,
+               omega_z=0.0, gamma=0.0, dt=1e-3 + 0j, nprocs=1, seed=11):
+    harm = tuple(L * L for L in box)   # pot = x^2 + y^2 + z^2, i.e. 2V with w = 1
+    d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                       Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=steps,
+                       omega_z=omega_z, gamma=gamma, dt_re=dt.real, dt_im=dt.imag,
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
+    psi0 = small_ic(shape, box, seed).astype(np.complex128 if prec == 64 else np.complex64)
+    r = refbuild.run(d, psi0, prec=prec, nprocs=nprocs)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi0=psi0, psi=r["psi"],
+                        t=r["t"], steps=steps, prec=prec, box=np.array(box),
+                        harm_w=np.array(harm), omega_z=omega_z, gamma=gamma,
+                        dt=np.complex128(dt), nprocs=nprocs)
+    return r
+
+
+def case_euler(name="euler_f64_16", steps=20):
+    """Reference imaginary time (src/run.f90:156-170) from its own 'soliton' initial
+    state (setup_thomas_fermi, src/initial.f90:65-77,637-652): no restart file."""
+    shape, box = (16, 16, 16), (16.0, 16.0, 16.0)
+    harm = (4.0, 4.0, 4.0)   # TF_dense = 1 - pot stays positive over part of the box
+    d = refbuild.build(name, prec=64, Nx=16, Ny=16, Nz=16, Lx=16.0, Ly=16.0, Lz=16.0,
+                       nsteps=0, shots=steps, imaginary_time=True, imag_nsteps=steps,
+                       imaginary_output=True, init_cond="soliton", dt_re=1e-3,
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
+    r = refbuild.run(d, None, prec=64, nprocs=1, shape=shape, snapshot="imag_var0001.dat")
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi=r["psi"], steps=steps,
+                        prec=64, box=np.array(box), harm_w=np.array(harm), dt=1e-3)
+
+
+def case_c1(nprocs=4, steps=1000, seed=20261203):
+    """Config 1 at full size: 64^3 FP64, 1000 steps, 4 ranks.  psi0 is regenerated from
+    trapbec_amd.initial.config_c1(seed); stored: its SHA-256, every 4th point of the
+    final psi, the z = 32 plane and the norms."""
+    pr, psi0 = initial.config_c1(seed)
+    kp = pr.kernel_params()
+    d = refbuild.build("c1_f64", prec=64, Nx=64, Ny=64, Nz=64, Lx=16.0, Ly=16.0, Lz=16.0,
+                       nsteps=steps, shots=steps, harm_wx=kp["harm_w"][0],
+                       harm_wy=kp["harm_w"][1], harm_wz=kp["harm_w"][2])
+    t0 = time.time()
+    r = refbuild.run(d, psi0, prec=64, nprocs=nprocs)
+    wall = time.time() - t0
+    psi = r["psi"]
+    np.savez_compressed(
+        os.path.join(OUT, "c1_f64_64.npz"),
+        psi0_sha256=hashlib.sha256(np.ascontiguousarray(psi0).tobytes()).hexdigest(),
+        sub=psi[::4, ::4, ::4], plane=psi[:, :, 32], steps=steps, seed=seed,
+        norm2=np.sum(np.abs(psi) ** 2), maxabs=np.abs(psi).max(), t=r["t"],
+        nprocs=nprocs, wall_s=wall)
+    return wall
+
+
+CASES = {
+    "rk3_f64_16": lambda: case_small("rk3_f64_16", 64),
+    "rk3_f64_16_rot": lambda: case_small("rk3_f64_16_rot", 64, omega_z=-1.4),
+    "rk3_f32_16": lambda: case_small("rk3_f32_16", 32),
+    "rk3_f32_16_rot": lambda: case_small("rk3_f32_16_rot", 32, omega_z=-1.4),
+    "rk3_f64_16_gamma": lambda: case_small("rk3_f64_16_gamma", 64, gamma=0.05),
+    "rk3_f64_16_imag": lambda: case_small("rk3_f64_16_imag", 64, dt=-1e-3j),
+    "rk3_f64_aniso": lambda: case_small("rk3_f64_aniso", 64, shape=(16, 12, 20),
+                                        box=(16.0, 12.0, 10.0), omega_z=-0.8),
+    "rk3_f64_16_np4": lambda: case_small("rk3_f64_16_np4", 64, omega_z=-1.4, nprocs=4),
+    "euler_f64_16": case_euler,
+    "c1_f64_64": case_c1,
+}
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(CASES)
+    for n in names:
+        t0 = time.time()
+        CASES[n]()
+        print(f"{n}: {time.time() - t0:.1f}s", flush=True)
