@@ -1,0 +1,214 @@
+"""Throughput of the fused RK3 GPE stepper on one MI355X.
+
+Workload: the 512^3 real-time GPE on [-16,16)^3 (h = 1/16), V = r^2/2, g = 500,
+psi0 = TF(mu = 10) x z-line vortex + 1e-3 seeded noise, dt = 5e-4, zero ghosts, Omega = 0.
+A step is one low-storage RK3 step (3 fused stage launches).  psi is resident in HBM
+before the timed region; K steps are replayed as captured graphs and timed with HIP
+events on the stepper's own stream, bracketed by device synchronisation.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu]
+
+Prints one JSON line.  `value` is FP32 (compensated stencil sums) Gpt-steps/s; the FP64
+figure is reported beside it.  Roofline: algorithmic bytes per grid-point step are 10
+complex transfers (80 B FP32, 160 B FP64): stage 1 reads psi, writes w and psi1; stage 2
+reads psi1 and w, writes w and psi2; stage 3 reads psi2 and w, writes psi.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+BYTES_PER_PT_STEP = {32: 80, 64: 160}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--seg", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    return ap.parse_args()
+
+
+def time_steps(torch, c, steps, warmup):
+    """Average ms per RK3 step over `steps` steps with HIP events on c's stream."""
+    from trapbec_amd import tbec
+    c.step(warmup, tbec.REAL_RK3)
+    c.sync()
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(c.stream_ptr())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    c.step(steps, tbec.REAL_RK3)
+    e1.record(stream)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def run_gpu(args, prec, steps, psi_dev, pr):
+    import torch
+    from trapbec_amd import tbec, autotune
+    c = tbec.Condensate.from_problem(pr, prec=prec)
+    tile, seg = autotune.lookup(pr.shape, prec)
+    if args.tile >= 0:
+        tile = args.tile
+    if args.seg > 0:
+        seg = args.seg
+    c.set_tuning(tile, seg)
+    c.upload_device(psi_dev.data_ptr())
+    c.sync()
+    ms, wall = time_steps(torch, c, steps, args.warmup)
+    npts = pr.shape[0] * pr.shape[1] * pr.shape[2]
+    gpts = npts / (ms * 1e-3) / 1e9
+    achieved = npts * BYTES_PER_PT_STEP[prec] / (ms * 1e-3)
+    c.close()
+    return {"ms_per_step": ms, "gpt_steps_per_s": gpts, "wall_s": wall,
+            "achieved_Bps": achieved, "tile": tile, "seg": seg,
+            "ms_per_launch": ms / 3.0}
+
+
+def cpu_baseline(n):
+    """The reference Fortran (oracle/_ref, built from its own sources) on this host's
+    cores, 512^3 FP32: wall time of a 3-step run minus a 1-step run of the same build
+    family, i.e. per-step time without start-up.  Falls back to the oracle's C
+    restatement (OpenMP) if the reference build or MPI is absent."""
+    import numpy as np  # noqa: F401
+    cores = min(16, os.cpu_count() or 1)
+    while cores > 1 and ((n % cores) and (n % max(1, cores // 2))):
+        cores //= 2
+    mpiexec = "/opt/conda/bin/mpiexec"
+    d1 = os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f32_s1")
+    d3 = os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f32_s3")
+    if os.path.exists(os.path.join(d1, "run.x")) and os.path.exists(
+            os.path.join(d3, "run.x")) and shutil.which(mpiexec):
+        times = {}
+        for d, s in ((d1, 1), (d3, 3)):
+            shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
+            os.makedirs(os.path.join(d, "data"))
+            for r in range(cores):
+                os.makedirs(os.path.join(d, "data", f"proc{r:03d}"), exist_ok=True)
+            t0 = time.perf_counter()
+            res = subprocess.run([mpiexec, "-n", str(cores), "./run.x"], cwd=d,
+                                 capture_output=True, text=True, errors="replace",
+                                 timeout=900)
+            times[s] = time.perf_counter() - t0
+            if res.returncode != 0:
+                raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+            shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
+        per_step = (times[3] - times[1]) / 2.0
+        return {"value": n ** 3 / per_step / 1e9, "unit": "Gpt-steps/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"reference Fortran FP32 {n}^3, mpiexec -n {cores}, wall(3 steps) "
+                          f"- wall(1 step) = {2 * per_step:.2f} s for 2 steps",
+                "s_per_step": per_step}
+    # fallback: the C restatement with OpenMP threads, 1 step at n^3
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import numpy as np
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    p = oracle.params((n, n, n), 32.0, (1024.0,) * 3, c0=0.0, gnl=1000.0, dt=5e-4)
+    s = oracle.Stepper(p, np.full((n, n, n), 0.1 + 0.05j, np.complex64), prec=32)
+    t0 = time.perf_counter()
+    s.rk3(1)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 / dt / 1e9, "unit": "Gpt-steps/s", "cores": cores,
+            "kind": "port", "sample": f"oracle C restatement FP32 {n}^3, 1 step, "
+                                      f"{cores} OpenMP threads", "s_per_step": dt}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the stepper "
+              "runs one condensate per GPU; multi-GPU sharding is a design note only)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from trapbec_amd import initial
+
+    n = args.n
+    print(f"[bench] building {n}^3 workload", file=sys.stderr, flush=True)
+    pr, psi0 = initial.config_m(n)
+    res = {}
+    for prec in (32, 64):
+        if args.only != "both" and args.only != f"f{prec}":
+            continue
+        cdt = torch.complex64 if prec == 32 else torch.complex128
+        psi_dev = torch.from_numpy(psi0.astype(np.complex64 if prec == 32 else np.complex128)
+                                   ).to("cuda:0")
+        assert psi_dev.dtype == cdt and psi_dev.is_contiguous()
+        steps = args.steps if prec == 32 else args.fp64_steps
+        print(f"[bench] fp{prec}: {steps} timed steps", file=sys.stderr, flush=True)
+        res[prec] = run_gpu(args, prec, steps, psi_dev, pr)
+        print(f"[bench] fp{prec}: {res[prec]['ms_per_step']:.3f} ms/step "
+              f"{res[prec]['gpt_steps_per_s']:.2f} Gpt-steps/s", file=sys.stderr, flush=True)
+        del psi_dev
+        torch.cuda.empty_cache()
+    head = 32 if 32 in res else 64
+    r = res[head]
+    out = {
+        "metric": "Gpt-steps/s at 512^3 FP32/FP64 (% HBM peak)",
+        "value": round(r["gpt_steps_per_s"], 4),
+        "unit": "Gpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps if head == 32 else args.fp64_steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == 32 else "f64",
+        "data": "synthetic (seeded TF + vortex + noise initial state)",
+        "config": {"workload": f"real-time GPE RK3, {n}^3, box [-16,16)^3, g=500, dt=5e-4, "
+                               f"zero ghosts, Omega=0",
+                   "grid": [n, n, n], "stencil": "6th-order FD, fused RK3 stage",
+                   "fp32_sum": "compensated", "tile": r["tile"], "seg": r["seg"]},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
+                     "traffic": None,
+                     "kernel": "gpe_stage (one RK3 stage)",
+                     "bytes_per_launch": n ** 3 * BYTES_PER_PT_STEP[head] / 3,
+                     "ms_per_launch": round(r["ms_per_launch"], 5)},
+    }
+    if 64 in res and head == 32:
+        f = res[64]
+        out["fp64"] = {"value": round(f["gpt_steps_per_s"], 4), "unit": "Gpt-steps/s",
+                       "ms_per_step": round(f["ms_per_step"], 5),
+                       "frac": round(f["achieved_Bps"] / HBM_PEAK, 4),
+                       "achieved_GBps": round(f["achieved_Bps"] / 1e9, 1),
+                       "steps": args.fp64_steps}
+    traffic = os.path.join(ROOT, "profiles", f"traffic_m{n}.json")
+    if os.path.exists(traffic):
+        with open(traffic) as fh:
+            out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch_fp32")
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        try:
+            out["cpu_baseline"] = cpu_baseline(n)
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"value": None, "error": str(e)[-500:]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+/* libtbec: MI355X (gfx950) GPE time stepper -- C ABI.
+ *
+ * The reference has no plugin interface: the hot path is the argument-less module
+ * procedure get_rhs() (src/finite_diff.f90:8-136) operating on module globals
+ * (src/cdata.f90:5-17) and the inline RK3 / imaginary-time array updates of
+ * program run (src/run.f90:156-192), configured at compile time by cparam.local
+ * (src/cparam.local:1-44).  libtbec replaces that path with a handle that owns the
+ * device state; the host program (trapbec_amd/driver, Fortran via iso_c_binding, or
+ * Python via ctypes) keeps the reference's driver role.
+ *
+ * Conventions: every function returns 0 on success and a negative TBEC_ERR_* code on
+ * failure; tbec_last_error() gives the message (thread-local).  Host psi buffers are
+ * interior-only complex arrays (interleaved re, im) of the handle's precision, with z
+ * fastest, then y, then x -- the reference's Psi(1:nz,1:ny,1:nx) (src/initial.f90:16).
+ * A handle is used by one host thread at a time; work is ordered on its own stream.
+ */
+#ifndef TBEC_H
+#define TBEC_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TBEC_OK = 0,
+  TBEC_ERR_ARG = -1,       /* invalid argument or parameter */
+  TBEC_ERR_OOM = -2,       /* device allocation failed */
+  TBEC_ERR_HIP = -3,       /* HIP runtime error */
+  TBEC_ERR_NONFINITE = -4, /* NaN/Inf detected in a reduction */
+  TBEC_ERR_STATE = -5      /* call not valid in the handle's current state */
+};
+
+enum { TBEC_F32 = 32, TBEC_F64 = 64 };
+
+/* Stencil summation: TBEC_SUM_REFERENCE keeps the reference's left-to-right 7-tap
+ * sum (src/deriv.f90:24); TBEC_SUM_COMPENSATED forms each symmetric pair with an
+ * error-free TwoSum and differences it against 2*f0 before weighting. */
+enum { TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1 };
+
+/* Step modes. */
+enum {
+  TBEC_REAL_RK3 = 0,       /* src/run.f90:179-192 (dt complex, times (1 - i*gamma)) */
+  TBEC_IMAG_RK3 = 1,       /* RK3 with dt = (0,-dtau), FP64 renormalisation each step,
+                              mu-shifted right-hand side (ground state is a fixed point) */
+  TBEC_IMAG_EULER_REF = 2  /* src/run.f90:157-159: psi += dt*0.5*rhs, no renormalisation */
+};
+
+/* Run parameters: the cparam.local entries the stepper reads (src/cparam.local:1-44),
+ * plus two coefficients that generalise the nonlinearity
+ *   rhs += (c0 - gnl*|psi|^2 - pot)*psi          (src/finite_diff.f90:132; c0=gnl=1 there)
+ * so the physical equation i psi_t = -lap/2 + V + g|psi|^2 - Omega L_z maps to
+ * c0 = 0, gnl = 2g, harm_w = w^2 L^2, omega_z = -2 Omega. */
+typedef struct {
+  int32_t nx, ny, nz;       /* Nx, Ny, Nz */
+  int32_t prec;             /* TBEC_F32 or TBEC_F64 */
+  int32_t sum_mode;         /* TBEC_SUM_* */
+  int32_t tile;             /* tile configuration id, -1 = library choice */
+  double Lx, Ly, Lz;        /* box; h = L/N, x_i = (i - 1/2) h - L/2 */
+  double harm_wx, harm_wy, harm_wz; /* pot = sum harm_w*(x/L)^2 (src/finite_diff.f90:34) */
+  double omega_z;           /* frame term omega_z*i*(x D1y - y D1x) (src/finite_diff.f90:128) */
+  double c0, gnl;
+  double dt_re, dt_im;      /* complex dt (src/cparam.local:4) */
+  double gamma;             /* dissipation: real-time dt -> dt*(1 - i*gamma) (src/run.f90:179) */
+} tbec_params;
+
+/* FP64 diagnostics of the current psi (dV = hx*hy*hz). */
+typedef struct {
+  double norm;       /* sum |psi|^2 dV */
+  double energy;     /* sum [psi* H psi - (gnl/4)|psi|^4] dV, H from the stepper's rhs */
+  double mu;         /* <psi|H|psi> / norm */
+  double residual;   /* ||H psi - mu psi|| / ||psi|| */
+  double max_abs;    /* maxPsi   (src/diagnostic.f90:10) */
+  double min_abs;    /* minPsi   (src/diagnostic.f90:12) */
+  double mean_abs;   /* meanPsi  (src/diagnostic.f90:14-16) */
+  double int_psi2;   /* intPsi2 = sum |psi|^2 without dV (src/diagnostic.f90:22) */
+} tbec_diag;
+
+typedef struct {
+  int64_t steps;        /* steps taken */
+  int32_t converged;    /* 1 if the relative energy change fell below tol */
+  int32_t pad_;
+  double energy, mu, residual, norm;
+  double rel_change;    /* |E_n - E_{n-1}| / |E_n| at the last step */
+} tbec_solve_result;
+
+typedef struct tbec_handle tbec_handle;
+
+/* Allocate device state for one condensate (src/initial.f90:16-25).  psi = 0. */
+int tbec_create(const tbec_params *p, tbec_handle **out);
+int tbec_destroy(tbec_handle *h);
+
+/* Copy psi in/out (replaces reload_data / var_print's Psi record,
+ * src/initial.f90:796-818, src/output.f90:103-107).  Host or device pointer variants;
+ * the device variant takes an interior-only contiguous device buffer. */
+int tbec_upload_psi(tbec_handle *h, const void *psi_host);
+int tbec_download_psi(tbec_handle *h, void *psi_host);
+int tbec_upload_psi_device(tbec_handle *h, const void *psi_dev);
+int tbec_download_psi_device(tbec_handle *h, void *psi_dev);
+
+/* Advance nsteps steps of the given mode (enqueued; returns before completion).
+ * Real time advances *t_inout by Re(dt) per step (src/run.f90:192). */
+int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout);
+
+/* One get_rhs evaluation of the current psi (src/finite_diff.f90:8-136) into a host
+ * buffer (interior layout).  Test hook. */
+int tbec_rhs(tbec_handle *h, void *rhs_host);
+
+/* Diagnostics of the current psi; synchronises. */
+int tbec_diagnose(tbec_handle *h, tbec_diag *out);
+
+/* Imaginary-time ground state: TBEC_IMAG_RK3 steps with dtau = -dt_im until
+ * |E_n - E_{n-1}| < tol*|E_n| or max_steps; renormalises to `norm` every step.  The
+ * host checks convergence every check_every steps (a captured graph in between). */
+int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
+                    int check_every, tbec_solve_result *out);
+
+/* Set the norm target used by TBEC_IMAG_RK3 steps (default 1). */
+int tbec_set_norm(tbec_handle *h, double norm);
+
+/* Tile shape id (0..3, per precision) and x-segment length used by the stage kernel;
+ * the autotuner (trapbec_amd/autotune.py) chooses them.  Drops captured graphs. */
+int tbec_set_tuning(tbec_handle *h, int tile, int seg);
+
+int tbec_sync(tbec_handle *h);
+/* The handle's HIP stream (hipStream_t), for event timing by the caller. */
+void *tbec_stream(tbec_handle *h);
+/* Average duration (ms) of one stage kernel measured with HIP events over n stages
+ * launched back to back on the handle's stream (real-time RK3 stages). */
+int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage);
+
+const char *tbec_last_error(void);
+/* Library version string and the gfx target it was built for. */
+const char *tbec_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

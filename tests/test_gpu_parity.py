@@ -1,0 +1,205 @@
+"""GPU parity: libtbec's fused HIP stages against the reference's own outputs (golden
+fixtures) and the CPU oracle on the same seeded inputs.
+
+Tolerances: FP64 psi within 1e-12 relative max-norm (after up to 1000 steps); FP32
+within the stated FP32 bounds.  Every case prints its name and shape before running."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, oracle
+from trapbec_amd import initial, tbec
+
+pytestmark = pytest.mark.gpu
+
+SMALL = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "rk3_*.npz")))
+
+
+def relmax(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def announce(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_rk3_vs_reference_fixture(name):
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    prec = int(f["prec"])
+    announce(f"{name} shape={f['psi0'].shape} prec={prec}")
+    box = tuple(float(b) for b in f["box"])
+    with tbec.Condensate(f["psi0"].shape, box, tuple(float(h) for h in f["harm_w"]),
+                         omega_z=float(f["omega_z"]), dt=complex(f["dt"]),
+                         gamma=float(f["gamma"]), prec=prec) as c:
+        c.upload(f["psi0"])
+        c.step(int(f["steps"]))
+        got = c.download()
+    tol = 1e-12 if prec == 64 else 2e-6
+    assert relmax(got, f["psi"]) < tol
+
+
+def test_euler_vs_reference_fixture():
+    f = np.load(os.path.join(GOLDEN, "euler_f64_16.npz"))
+    announce("euler_f64_16")
+    box = tuple(float(b) for b in f["box"])
+    harm = tuple(float(h) for h in f["harm_w"])
+    psi0 = initial.reference_soliton((16, 16, 16), box, harm)
+    with tbec.Condensate((16, 16, 16), box, harm, dt=float(f["dt"]), prec=64) as c:
+        c.upload(psi0)
+        c.step(int(f["steps"]), tbec.IMAG_EULER_REF)
+        got = c.download()
+    assert relmax(got, f["psi"]) < 1e-12
+
+
+def test_c1_full_size_vs_reference():
+    """Config 1: 64^3 FP64, 1000 steps, against the 4-rank reference run and the oracle."""
+    f = np.load(os.path.join(GOLDEN, "c1_f64_64.npz"))
+    announce("c1 64^3 fp64 1000 steps")
+    pr, psi0 = initial.config_c1(int(f["seed"]))
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        c.step(int(f["steps"]))
+        got = c.download()
+    assert relmax(got[::4, ::4, ::4], f["sub"]) < 1e-12
+    assert relmax(got[:, :, 32], f["plane"]) < 1e-12
+    kp = pr.kernel_params()
+    s = oracle.Stepper(oracle.params(pr.shape, pr.box, kp["harm_w"], dt=1e-3), psi0)
+    s.rk3(int(f["steps"]))
+    assert relmax(got, s.interior) < 1e-12
+
+
+def _rand_psi(shape, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return scale * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+
+@pytest.mark.parametrize("shape,rot", [((37, 23, 50), 0.9), ((70, 19, 131), 0.0),
+                                       ((8, 8, 8), -0.5)])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ragged_grids_vs_oracle(shape, rot, prec):
+    """Grids that are not multiples of any tile (masked stores, zero padding reads)."""
+    announce(f"ragged {shape} rot={rot} prec={prec}")
+    box = (7.0, 5.0, 9.0)
+    harm = (3.0, 2.0, 5.0)
+    psi0 = _rand_psi(shape, 3, 0.3)
+    p = oracle.params(shape, box, harm, omega_z=rot, c0=0.7, gnl=1.3, dt=2e-4)
+    s = oracle.Stepper(p, psi0.astype(np.complex128 if prec == 64 else np.complex64), prec=prec)
+    r_ref = s.get_rhs()
+    s.rk3(5)
+    with tbec.Condensate(shape, box, harm, omega_z=rot, c0=0.7, gnl=1.3, dt=2e-4,
+                         prec=prec) as c:
+        c.upload(psi0)
+        r = c.rhs()
+        c.step(5)
+        got = c.download()
+    if prec == 64:
+        assert relmax(r, r_ref) < 1e-12
+        assert relmax(got, s.interior) < 1e-12
+    else:
+        assert relmax(r, r_ref) < 1e-5
+        assert relmax(got, s.interior) < 1e-5
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_every_tile_shape(tile, prec):
+    shape = (40, 72, 136)
+    announce(f"tile {tile} prec {prec} {shape}")
+    psi0 = _rand_psi(shape, 9, 0.2)
+    harm = (4.0, 4.0, 9.0)
+    p = oracle.params(shape, 8.0, harm, omega_z=0.4, dt=1e-4)
+    s = oracle.Stepper(p, psi0.astype(np.complex128 if prec == 64 else np.complex64), prec=prec)
+    s.rk3(3)
+    with tbec.Condensate(shape, 8.0, harm, omega_z=0.4, dt=1e-4, prec=prec, tile=tile) as c:
+        c.upload(psi0)
+        c.step(3)
+        got = c.download()
+    assert relmax(got, s.interior) < (1e-12 if prec == 64 else 1e-5)
+
+
+def test_diagnostics_vs_oracle():
+    shape, box = (24, 32, 20), (8.0, 9.0, 7.0)
+    announce(f"diagnose {shape}")
+    psi0 = initial.tf_profile(shape, box, (1.0, 1.2, 0.9), 10.0, mu=4.0)[0] * \
+        initial.vortex_line_factor(shape, box, 0.2, -0.1, 1, 0.5)
+    pr = initial.Problem(shape, box, (1.0, 1.2, 0.9), 10.0, 0.3, 1e-3)
+    kp = pr.kernel_params()
+    s = oracle.Stepper(oracle.params(shape, box, kp["harm_w"], omega_z=kp["omega_r"],
+                                     c0=kp["c0"], gnl=kp["gnl"]), psi0)
+    e = s.energy()
+    info = s.info()
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        d = c.diagnose()
+    for k in ("norm", "energy", "mu"):
+        assert d[k] == pytest.approx(e[k], rel=1e-12), k
+    assert d["residual"] == pytest.approx(e["residual"], rel=1e-8)
+    assert d["max_abs"] == pytest.approx(info[0], rel=1e-14)
+    assert d["min_abs"] == pytest.approx(info[1], abs=1e-14)
+    assert d["mean_abs"] == pytest.approx(info[2], rel=1e-12)
+    assert d["int_psi2"] == pytest.approx(info[3], rel=1e-12)
+
+
+def test_imag_rk3_renorm_vs_oracle():
+    """Renormalised, mu-shifted imaginary-time RK3 steps match the FP64 oracle."""
+    shape, box, om, g = (24, 24, 24), (12.0, 12.0, 12.0), (1.0, 1.1, 0.9), 20.0
+    announce(f"imag rk3 {shape}")
+    pr = initial.Problem(shape, box, om, g, 0.3, -5e-3j)
+    kp = pr.kernel_params()
+    psi0 = initial.tf_profile(shape, box, om, g, norm=1.0)[0] * \
+        initial.phase_noise(shape, 5, 0.1)
+    s = oracle.Stepper(oracle.params(shape, box, kp["harm_w"], omega_z=kp["omega_r"],
+                                     c0=kp["c0"], gnl=kp["gnl"], dt=kp["dt"]), psi0)
+    hist = s.imag_rk3(40, 1.0)
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        c.step(40, tbec.IMAG_RK3)
+        got = c.download()
+        d = c.diagnose()
+    assert relmax(got, s.interior) < 1e-12
+    assert d["norm"] == pytest.approx(1.0, rel=1e-12)
+    e = s.energy()
+    assert d["energy"] == pytest.approx(e["energy"], rel=1e-12)
+    assert len(hist) == 40
+
+
+def test_imag_solve_converges_harmonic():
+    """g = 0 isotropic trap from a displaced Gaussian: converges to E = 3/2 (FP64)."""
+    shape, box = (32, 32, 32), (12.0, 12.0, 12.0)
+    announce("imag solve harmonic")
+    pr = initial.Problem(shape, box, (1.0, 1.0, 1.0), 0.0, 0.0, -1e-2j)
+    X, Y, Z = initial.mesh(shape, box)
+    psi0 = np.exp(-0.4 * ((X - 0.5) ** 2 + Y ** 2 + Z ** 2)).astype(complex)
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        r = c.imag_solve(20000, 1e-12, 1.0, 100)
+        d = c.diagnose()
+    assert r["converged"] == 1
+    assert r["energy"] == pytest.approx(1.5, rel=2e-5)
+    assert d["energy"] == pytest.approx(r["energy"], rel=1e-10)
+    assert d["residual"] < 1e-3
+
+
+def test_fp32_compensated_vs_fp64():
+    """FP32 with compensated stencil sums tracks FP64 on a 128^3 slice of the throughput
+    workload over 200 steps (relative L2 < 1e-5) and beats the uncompensated FP32 sum."""
+    n = 128
+    announce(f"fp32 vs fp64 {n}^3")
+    pr, psi0 = initial.config_m(n)
+    out = {}
+    for prec, sm in ((64, tbec.SUM_REFERENCE), (32, tbec.SUM_COMPENSATED),
+                     (32, tbec.SUM_REFERENCE)):
+        with tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm) as c:
+            c.upload(psi0)
+            c.step(200)
+            out[(prec, sm)] = c.download().astype(np.complex128)
+    ref = out[(64, 0)]
+    l2 = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))  # noqa: E731
+    e_comp, e_naive = l2(out[(32, 1)]), l2(out[(32, 0)])
+    print(f"fp32 rel L2: compensated {e_comp:.3e}  reference-order {e_naive:.3e}")
+    assert e_comp < 1e-5
+    assert e_comp < e_naive

@@ -1,0 +1,65 @@
+"""Tile-shape autotuner for the fused stage kernel.
+
+The stage kernel is compiled for four (TZ, TY, VZ) tile shapes per precision and takes
+the x-segment length SX at run time.  `tune` times every (tile, SX) candidate on the
+GPU for one grid shape with HIP events (libtbec's tbec_time_stages) and records the
+fastest in trapbec_amd/tuning.json; `lookup` returns the recorded choice, else a
+default.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+DEFAULT = {32: (0, 64), 64: (0, 64)}
+SEGS = (16, 32, 64, 128)
+
+
+def _key(shape, prec):
+    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}"
+
+
+def _load():
+    if os.path.exists(TABLE):
+        with open(TABLE) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(shape, prec):
+    t = _load().get(_key(shape, prec))
+    if t:
+        return int(t["tile"]), int(t["seg"])
+    return DEFAULT[prec]
+
+
+def tune(shape, prec, psi0=None, nstages=30, save=True, verbose=False):
+    """Time every (tile, seg) for this grid and precision; return the table entry."""
+    import numpy as np
+    from trapbec_amd import tbec
+    c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
+                        dt=1e-4, prec=prec)
+    if psi0 is None:
+        psi0 = np.full(shape, 0.1 + 0.05j)
+    c.upload(psi0)
+    results = []
+    for tile in range(4):
+        for seg in SEGS:
+            c.set_tuning(tile, seg)
+            c.time_stages(3)
+            ms = c.time_stages(nstages)
+            results.append((ms, tile, seg))
+            if verbose:
+                print(f"tile {tile} seg {seg}: {ms * 1e3:.1f} us/stage", flush=True)
+    c.close()
+    ms, tile, seg = min(results)
+    entry = {"tile": tile, "seg": seg, "us_per_stage": ms * 1e3,
+             "all": [[t, s, round(m * 1e3, 2)] for m, t, s in sorted(results)]}
+    if save:
+        table = _load()
+        table[_key(shape, prec)] = entry
+        with open(TABLE, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    return entry

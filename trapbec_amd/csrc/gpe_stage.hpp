@@ -1,0 +1,412 @@
+// Fused GPE right-hand side + low-storage RK3 stage for gfx950 (CDNA4).
+//
+// One launch = one RK stage over the whole grid: it reads psi once from HBM, evaluates
+//   rhs = D2x psi + D2y psi + D2z psi + om*i*(x D1y psi - y D1x psi)
+//         + (c0 - gnl*|psi|^2 - pot) psi,     pot = sum_a harm_w[a]*(x_a/L_a)^2
+// (src/finite_diff.f90:19-135, src/deriv.f90:12-25) and applies the stage update
+//   w <- A*w + c*rhs,  psi_out <- psi + B*w     (c = i*dt/2)
+// (src/run.f90:183-191), writing psi_out and w.  Variants: the reference imaginary
+// Euler update psi + (dt/2)*rhs (src/run.f90:159) and a rhs-only pass (test hook).
+//
+// Layout (per condensate): psi, w are padded arrays, z fastest.  Interior (0,0,0) sits
+// at `base`; strides sy (y) and sx (x) in complex elements.  The halo (3 cells, more at
+// row ends) is zero and never written, which is the reference's zero boundary
+// (src/ghost.f90:55,88,121,154,179-181).
+//
+// Work decomposition: a workgroup owns a TZ x TY tile of (z, y) and marches along x
+// over a segment of SX planes.  Each thread owns VZ consecutive z points of one y row
+// and keeps their x-neighbourhood (7 planes) in a register queue, so D2x/D1x need no
+// memory traffic.  The centre plane of the tile plus a 3-deep y halo and a 4-wide
+// (aligned) z halo is staged in LDS (double buffered, one barrier per plane) for the y
+// and z stencils.  Plane x+4 (own points) and plane x+1 (halo) are prefetched one
+// iteration ahead.  Consecutive workgroup ids are spread over the 8 XCDs by the
+// dispatcher; the tile index is remapped so each XCD gets a contiguous block of tiles
+// (shared halos stay in that XCD's L2).
+//
+// Summation modes for the 7-tap D2 sums:
+//   SUM_REF  : the reference's left-to-right sum (FP64 parity path)
+//   SUM_COMP : compensated: each symmetric pair f[-k]+f[k] is formed by an error-free
+//              TwoSum (s, e); d_k = (s - 2 f0) + e (the subtraction is exact by
+//              Sterbenz for smooth data), D2 = (270 d1 - 27 d2 + 2 d3)/(180 h^2).
+//              The rounding is then relative to h^2 f'' instead of to f.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tbec {
+
+enum SumMode { SUM_REF = 0, SUM_COMP = 1 };
+enum StageKind { ST_RK1 = 0, ST_RK2 = 1, ST_RK3 = 2, ST_EULER = 3, ST_RHS = 4 };
+
+template <typename T> struct alignas(2 * sizeof(T)) cplx { T re, im; };
+
+struct StageArgs {
+  const void *psi_in;
+  void *psi_out;
+  void *w;
+  int nx, ny, nz;
+  int seg_len;                 // SX
+  int tiles_z, tiles_y, nseg;  // tile grid
+  int nblocks;                 // tiles_z * tiles_y * nseg
+  long long sy, sx, base;      // strides / interior offset (complex elements)
+  double x0, y0, z0;           // coordinate of interior index 0 on each axis
+  double hx, hy, hz;
+  double k2[3];                // 1 / (180 h_a^2)
+  double k1[2];                // 1 / (60 h_x), 1 / (60 h_y)
+  double wl[3];                // harm_w[a] / L_a^2
+  double c0, gnl, om;
+  double cre, cim;             // complex update coefficient
+  double A, B;                 // stage coefficients
+  int stage;                   // StageKind
+  const double *scale;         // imaginary time: psi is read as scale*psi (or null)
+  const double *c0dev;         // imaginary time: c0 from device memory (or null)
+  const int *skip;             // imaginary time: converged flag; skip the pass when set
+  double *partials;            // reductions: 4 doubles per workgroup
+};
+
+// ---- small helpers -------------------------------------------------------------
+template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
+  s = a + b;
+  T bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+template <typename T, int VZ> struct VecIO;
+template <> struct VecIO<float, 4> {   // 4 complex floats = 2 x 16 B
+  static __device__ __forceinline__ void ld(const cplx<float> *p, cplx<float> *o) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+    float4 a = q[0], b = q[1];
+    o[0] = {a.x, a.y}; o[1] = {a.z, a.w}; o[2] = {b.x, b.y}; o[3] = {b.z, b.w};
+  }
+  static __device__ __forceinline__ void st(cplx<float> *p, const cplx<float> *o) {
+    float4 *q = reinterpret_cast<float4 *>(p);
+    q[0] = make_float4(o[0].re, o[0].im, o[1].re, o[1].im);
+    q[1] = make_float4(o[2].re, o[2].im, o[3].re, o[3].im);
+  }
+};
+template <> struct VecIO<float, 2> {
+  static __device__ __forceinline__ void ld(const cplx<float> *p, cplx<float> *o) {
+    float4 a = *reinterpret_cast<const float4 *>(p);
+    o[0] = {a.x, a.y}; o[1] = {a.z, a.w};
+  }
+  static __device__ __forceinline__ void st(cplx<float> *p, const cplx<float> *o) {
+    *reinterpret_cast<float4 *>(p) = make_float4(o[0].re, o[0].im, o[1].re, o[1].im);
+  }
+};
+template <> struct VecIO<double, 2> {  // 2 complex doubles = 2 x 16 B
+  static __device__ __forceinline__ void ld(const cplx<double> *p, cplx<double> *o) {
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+    double2 a = q[0], b = q[1];
+    o[0] = {a.x, a.y}; o[1] = {b.x, b.y};
+  }
+  static __device__ __forceinline__ void st(cplx<double> *p, const cplx<double> *o) {
+    double2 *q = reinterpret_cast<double2 *>(p);
+    q[0] = make_double2(o[0].re, o[0].im);
+    q[1] = make_double2(o[1].re, o[1].im);
+  }
+};
+template <> struct VecIO<double, 1> {
+  static __device__ __forceinline__ void ld(const cplx<double> *p, cplx<double> *o) {
+    double2 a = *reinterpret_cast<const double2 *>(p);
+    o[0] = {a.x, a.y};
+  }
+  static __device__ __forceinline__ void st(cplx<double> *p, const cplx<double> *o) {
+    *reinterpret_cast<double2 *>(p) = make_double2(o[0].re, o[0].im);
+  }
+};
+
+// 7-tap second derivative sum (without the 1/(180 h^2) factor) of one component.
+template <typename T, int SUM>
+__device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
+  if constexpr (SUM == SUM_REF) {
+    return (((((T(2) * m3 - T(27) * m2) + T(270) * m1) - T(490) * c) + T(270) * p1) -
+            T(27) * p2) + T(2) * p3;
+  } else {
+    T s1, e1, s2, e2, s3, e3;
+    two_sum(m1, p1, s1, e1);
+    two_sum(m2, p2, s2, e2);
+    two_sum(m3, p3, s3, e3);
+    const T c2 = c + c;
+    const T d1 = (s1 - c2) + e1, d2 = (s2 - c2) + e2, d3 = (s3 - c2) + e3;
+    return T(270) * d1 - T(27) * d2 + T(2) * d3;
+  }
+}
+
+// 7-tap first derivative sum (without 1/(60 h)).
+template <typename T>
+__device__ __forceinline__ T d1sum(T m3, T m2, T m1, T p1, T p2, T p3) {
+  return ((((-m3 + T(9) * m2) - T(45) * m1) + T(45) * p1) - T(9) * p2) + p3;
+}
+
+// Workgroup-wide sum of 4 doubles; thread 0 writes them.
+template <int NT>
+__device__ __forceinline__ void block_sum4(double v[4], double *out) {
+  __shared__ double red[NT / 64][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wid][q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+      for (int k = 0; k < NT / 64; ++k) s += red[k][q];
+      out[q] = s;
+    }
+  }
+}
+
+// ---- the stage kernel ----------------------------------------------------------
+template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+__global__ void __launch_bounds__((TZ / VZ) * TY)
+gpe_stage(const StageArgs a) {
+  using C = cplx<T>;
+  constexpr int NTZ = TZ / VZ;
+  constexpr int NT = NTZ * TY;
+  constexpr int HZV = 4 / VZ;           // vectors per z-halo side (4 complex wide)
+  constexpr int NHALO = 6 * NTZ + 2 * TY * HZV;
+  static_assert(NHALO <= NT, "one halo vector per thread at most");
+  constexpr int LZ = TZ + 8;            // 4 halo columns each side
+  constexpr int LY = TY + 6;
+  __shared__ C lds[2][LY][LZ];
+
+  if constexpr (IMAG) {
+    if (a.skip && *a.skip) return;
+  }
+
+  // XCD-aware tile order: hardware id b lands on XCD b % 8; give XCD k the logical
+  // range [k*per, (k+1)*per).
+  const int nb = a.nblocks;
+  const int b = blockIdx.x;
+  int lb = b;
+  if ((nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
+  const int tiles = a.tiles_z * a.tiles_y;
+  const int seg = lb / tiles;
+  const int t2 = lb - seg * tiles;
+  const int tyi = t2 / a.tiles_z;
+  const int tzi = t2 - tyi * a.tiles_z;
+  const int zb = tzi * TZ, yb = tyi * TY;
+  const int xs = seg * a.seg_len;
+  const int xe = min(xs + a.seg_len, a.nx);
+
+  const int tid = threadIdx.x;
+  const int tz = tid % NTZ, ty = tid / NTZ;
+  const int z = zb + tz * VZ, y = yb + ty;
+
+  const C *__restrict__ pin = reinterpret_cast<const C *>(a.psi_in) + a.base;
+  C *__restrict__ pout = reinterpret_cast<C *>(a.psi_out) + a.base;
+  C *__restrict__ pw = reinterpret_cast<C *>(a.w) + a.base;
+  const long long sx = a.sx, sy = a.sy;
+
+  T sc = T(1);
+  T c0 = T(a.c0);
+  if constexpr (IMAG) {
+    if (a.scale) sc = T(*a.scale);
+    if (a.c0dev) c0 = T(*a.c0dev);
+  }
+
+  // Own column pointer (x index added per plane).  Rows/columns beyond the interior
+  // read zeros from the padding; their results are never stored.
+  const long long own = (long long)y * sy + z;
+
+  // Halo assignment: vectors [0, 6*NTZ) are y-halo rows, the rest z-halo columns.
+  long long hoff = 0;
+  int hrow = 0, hcol = 0;
+  const bool hl = tid < NHALO;
+  if (hl) {
+    if (tid < 6 * NTZ) {
+      const int r = tid / NTZ, cz = tid - r * NTZ;
+      const int dy = r < 3 ? r - 3 : TY + r - 3;
+      hrow = 3 + dy;
+      hcol = 4 + cz * VZ;
+      hoff = (long long)(yb + dy) * sy + (zb + cz * VZ);
+    } else {
+      const int q = tid - 6 * NTZ;            // [0, 2*TY*HZV)
+      const int side = q / (TY * HZV);
+      const int r2 = q - side * TY * HZV;
+      const int row = r2 / HZV, v = r2 - row * HZV;
+      hrow = 3 + row;
+      const int zc = side == 0 ? (-4 + v * VZ) : (TZ + v * VZ);
+      hcol = 4 + zc;
+      hoff = (long long)(yb + row) * sy + (zb + zc);
+    }
+  }
+
+  // register queue: q[0..6] = planes x-3..x+3 of the own VZ points
+  C q[7][VZ];
+  C nxt[VZ];
+  C hcur[VZ], hnxt[VZ];
+  auto ldv = [&](const C *p, C *o) {
+    VecIO<T, VZ>::ld(p, o);
+    if constexpr (IMAG) {
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) { o[v].re *= sc; o[v].im *= sc; }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 7; ++k) ldv(pin + (long long)(xs - 3 + k) * sx + own, q[k]);
+  if (hl) ldv(pin + (long long)xs * sx + hoff, hcur);
+
+  const T k2x = T(a.k2[0]), k2y = T(a.k2[1]), k2z = T(a.k2[2]);
+  const T k1x = T(a.k1[0]), k1y = T(a.k1[1]);
+  const T wlx = T(a.wl[0]), wly = T(a.wl[1]), wlz = T(a.wl[2]);
+  const T gnl = T(a.gnl), om = T(a.om);
+  const T cre = T(a.cre), cim = T(a.cim), A = T(a.A), B = T(a.B);
+  const int stage = a.stage;
+  const T yc = T(a.y0 + a.hy * y);
+  T zc[VZ];
+#pragma unroll
+  for (int v = 0; v < VZ; ++v) zc[v] = T(a.z0 + a.hz * (z + v));
+  bool zin[VZ];
+#pragma unroll
+  for (int v = 0; v < VZ; ++v) zin[v] = (z + v) < a.nz;
+  const bool yin = y < a.ny;
+
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+
+  for (int x = xs; x < xe; ++x) {
+    const int buf = (x - xs) & 1;
+    // prefetch plane x+4 (own) and x+1 (halo)
+    ldv(pin + (long long)(x + 4) * sx + own, nxt);
+    if (hl) ldv(pin + (long long)(x + 1) * sx + hoff, hnxt);
+    C wv[VZ];
+    if (stage == ST_RK2 || stage == ST_RK3) VecIO<T, VZ>::ld(pw + (long long)x * sx + own, wv);
+
+    // stage centre plane + halo into LDS
+    {
+      C *row = &lds[buf][3 + ty][4 + tz * VZ];
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) row[v] = q[3][v];
+      if (hl) {
+        C *hr = &lds[buf][hrow][hcol];
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) hr[v] = hcur[v];
+      }
+    }
+    __syncthreads();
+
+    const T xc = T(a.x0 + a.hx * x);
+    const T potxy = wlx * (xc * xc) + wly * (yc * yc);
+    // z row segment: columns tz*VZ .. tz*VZ+VZ+7  (z-4 .. z+VZ+3)
+    C zr[VZ + 8];
+    {
+      const C *row = &lds[buf][3 + ty][tz * VZ];
+#pragma unroll
+      for (int k = 0; k < VZ + 8; ++k) zr[k] = row[k];
+    }
+    C out[VZ], wo[VZ];
+#pragma unroll
+    for (int v = 0; v < VZ; ++v) {
+      const int col = 4 + tz * VZ + v;
+      const C ym3 = lds[buf][ty][col], ym2 = lds[buf][ty + 1][col],
+              ym1 = lds[buf][ty + 2][col], yp1 = lds[buf][ty + 4][col],
+              yp2 = lds[buf][ty + 5][col], yp3 = lds[buf][ty + 6][col];
+      const C f = q[3][v];
+      const C zm3 = zr[v + 1], zm2 = zr[v + 2], zm1 = zr[v + 3], zp1 = zr[v + 5],
+              zp2 = zr[v + 6], zp3 = zr[v + 7];
+      T rr, ri;
+      {
+        const T dxr = d2sum<T, SUM>(q[0][v].re, q[1][v].re, q[2][v].re, f.re, q[4][v].re,
+                                    q[5][v].re, q[6][v].re);
+        const T dxi = d2sum<T, SUM>(q[0][v].im, q[1][v].im, q[2][v].im, f.im, q[4][v].im,
+                                    q[5][v].im, q[6][v].im);
+        const T dyr = d2sum<T, SUM>(ym3.re, ym2.re, ym1.re, f.re, yp1.re, yp2.re, yp3.re);
+        const T dyi = d2sum<T, SUM>(ym3.im, ym2.im, ym1.im, f.im, yp1.im, yp2.im, yp3.im);
+        const T dzr = d2sum<T, SUM>(zm3.re, zm2.re, zm1.re, f.re, zp1.re, zp2.re, zp3.re);
+        const T dzi = d2sum<T, SUM>(zm3.im, zm2.im, zm1.im, f.im, zp1.im, zp2.im, zp3.im);
+        rr = (dxr * k2x + dyr * k2y) + dzr * k2z;
+        ri = (dxi * k2x + dyi * k2y) + dzi * k2z;
+      }
+      if constexpr (ROT) {
+        const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
+        const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
+        const T d1xr = d1sum<T>(q[0][v].re, q[1][v].re, q[2][v].re, q[4][v].re, q[5][v].re,
+                                q[6][v].re) * k1x;
+        const T d1xi = d1sum<T>(q[0][v].im, q[1][v].im, q[2][v].im, q[4][v].im, q[5][v].im,
+                                q[6][v].im) * k1x;
+        const T pr = xc * d1yr - yc * d1xr;
+        const T pim = xc * d1yi - yc * d1xi;
+        rr -= om * pim;   // (0, om) * (pr, pim)
+        ri += om * pr;
+      }
+      const T n2 = f.re * f.re + f.im * f.im;
+      const T pot = potxy + wlz * (zc[v] * zc[v]);
+      const T fac = (c0 - gnl * n2) - pot;
+      rr += fac * f.re;
+      ri += fac * f.im;
+
+      const T ur = cre * rr - cim * ri, ui = cre * ri + cim * rr;
+      if (stage == ST_RHS) {
+        wo[v] = {rr, ri};
+      } else if (stage == ST_RK1 || stage == ST_EULER) {
+        wo[v] = {ur, ui};
+        out[v] = {f.re + B * ur, f.im + B * ui};
+      } else {
+        wo[v] = {A * wv[v].re + ur, A * wv[v].im + ui};
+        out[v] = {f.re + B * wo[v].re, f.im + B * wo[v].im};
+      }
+      if constexpr (IMAG) {
+        if (yin && zin[v]) {
+          if (stage == ST_RK1 || stage == ST_RHS) {
+            red[0] += (double)n2;
+            red[1] += (double)f.re * (double)rr + (double)f.im * (double)ri;
+            red[2] += (double)n2 * (double)n2;
+            red[3] += (double)rr * (double)rr + (double)ri * (double)ri;
+          } else if (stage == ST_RK3 || stage == ST_EULER) {
+            red[0] += (double)out[v].re * (double)out[v].re +
+                      (double)out[v].im * (double)out[v].im;
+          }
+        }
+      }
+    }
+    // stores (interior points only; padding stays zero)
+    if (yin) {
+      const long long o = (long long)x * sx + own;
+      bool full = true;
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) full = full && zin[v];
+      if (full) {
+        if (stage == ST_RHS) {
+          VecIO<T, VZ>::st(pw + o, wo);
+        } else {
+          VecIO<T, VZ>::st(pout + o, out);
+          if (stage == ST_RK1 || stage == ST_RK2) VecIO<T, VZ>::st(pw + o, wo);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) {
+          if (!zin[v]) continue;
+          if (stage == ST_RHS) {
+            pw[o + v] = wo[v];
+          } else {
+            pout[o + v] = out[v];
+            if (stage == ST_RK1 || stage == ST_RK2) pw[o + v] = wo[v];
+          }
+        }
+      }
+    }
+    // rotate the queue
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) q[k][v] = q[k + 1][v];
+    }
+#pragma unroll
+    for (int v = 0; v < VZ; ++v) {
+      q[6][v] = nxt[v];
+      hcur[v] = hnxt[v];
+    }
+  }
+  if constexpr (IMAG) {
+    if (a.partials) block_sum4<NT>(red, a.partials + 4LL * lb);
+  }
+}
+
+}  // namespace tbec

@@ -1,0 +1,677 @@
+// libtbec: handle, device layout, launch plumbing and reductions for the fused GPE
+// stage kernel (gpe_stage.hpp).  C ABI in include/tbec.h.
+#include "tbec.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+
+#include "gpe_stage.hpp"
+#include "tbec_internal.hpp"
+
+using namespace tbec;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(TBEC_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                             \
+  } while (0)
+
+namespace tbec {
+
+int set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------- tile table ----
+// Each precision has a small set of tile shapes (TZ x TY x VZ); the segment length SX
+// is a runtime parameter.  The autotuner (trapbec_amd/autotune.py) picks (tile, SX).
+const TileCfg kTilesF32[] = {{64, 16, 4}, {64, 32, 4}, {128, 8, 4}, {32, 32, 4}};
+const TileCfg kTilesF64[] = {{32, 16, 2}, {64, 8, 2}, {32, 32, 2}, {16, 16, 2}};
+const int kNumTiles = 4;
+
+const TileCfg &tile_cfg(int prec, int id) {
+  return prec == TBEC_F32 ? kTilesF32[id] : kTilesF64[id];
+}
+
+template <typename T, int TZ, int TY, int VZ>
+static hipError_t launch_tile(const StageArgs &a, int sum, bool rot, bool imag,
+                              hipStream_t s) {
+  constexpr int NT = (TZ / VZ) * TY;
+  dim3 grid(a.nblocks), block(NT);
+#define TBEC_L(S, R, I) \
+  hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, R, I>), grid, block, 0, s, a)
+  if (sum == SUM_REF) {
+    if (rot) { if (imag) TBEC_L(SUM_REF, true, true); else TBEC_L(SUM_REF, true, false); }
+    else { if (imag) TBEC_L(SUM_REF, false, true); else TBEC_L(SUM_REF, false, false); }
+  } else {
+    if (rot) { if (imag) TBEC_L(SUM_COMP, true, true); else TBEC_L(SUM_COMP, true, false); }
+    else { if (imag) TBEC_L(SUM_COMP, false, true); else TBEC_L(SUM_COMP, false, false); }
+  }
+#undef TBEC_L
+  return hipGetLastError();
+}
+
+hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool rot,
+                        bool imag, hipStream_t s) {
+  if (prec == TBEC_F32) {
+    switch (tile) {
+      case 0: return launch_tile<float, 64, 16, 4>(a, sum, rot, imag, s);
+      case 1: return launch_tile<float, 64, 32, 4>(a, sum, rot, imag, s);
+      case 2: return launch_tile<float, 128, 8, 4>(a, sum, rot, imag, s);
+      default: return launch_tile<float, 32, 32, 4>(a, sum, rot, imag, s);
+    }
+  } else {
+    switch (tile) {
+      case 0: return launch_tile<double, 32, 16, 2>(a, sum, rot, imag, s);
+      case 1: return launch_tile<double, 64, 8, 2>(a, sum, rot, imag, s);
+      case 2: return launch_tile<double, 32, 32, 2>(a, sum, rot, imag, s);
+      default: return launch_tile<double, 16, 16, 2>(a, sum, rot, imag, s);
+    }
+  }
+}
+
+// ------------------------------------------------------------- small kernels ----
+template <typename T>
+__global__ void k_unpack(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst, int nx,
+                         int ny, int nz, long long sy, long long sx, long long base) {
+  const long long n = (long long)nx * ny * nz;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i % nz, r = i / nz, y = r % ny, x = r / ny;
+    dst[base + x * sx + y * sy + z] = src[i];
+  }
+}
+
+template <typename T>
+__global__ void k_pack(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst, int nx,
+                       int ny, int nz, long long sy, long long sx, long long base,
+                       const double *scale) {
+  const long long n = (long long)nx * ny * nz;
+  const T s = scale ? T(*scale) : T(1);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i % nz, r = i / nz, y = r % ny, x = r / ny;
+    cplx<T> v = src[base + x * sx + y * sy + z];
+    if (scale) { v.re *= s; v.im *= s; }
+    dst[i] = v;
+  }
+}
+
+// psi *= scale over the interior, then scale = 1.
+template <typename T>
+__global__ void k_apply_scale(cplx<T> *__restrict__ p, int nx, int ny, int nz, long long sy,
+                              long long sx, long long base, const double *scale) {
+  const long long n = (long long)nx * ny * nz;
+  const T s = T(*scale);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i % nz, r = i / nz, y = r % ny, x = r / ny;
+    cplx<T> &v = p[base + x * sx + y * sy + z];
+    v.re *= s;
+    v.im *= s;
+  }
+}
+
+__global__ void k_reset_scale(double *st) { st[ST_SCALE] = 1.0; }
+
+// Psi_info partials: max|psi|, min|psi|, sum|psi|, sum|psi|^2 (src/diagnostic.f90:7-25).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_info(const cplx<T> *__restrict__ p, int nx, int ny, int nz, long long sy, long long sx,
+       long long base, const double *scale, double *part) {
+  const long long n = (long long)nx * ny * nz;
+  const double s = scale ? *scale : 1.0;
+  double mx = 0.0, mn = INFINITY, s1 = 0.0, s2 = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long z = i % nz, r = i / nz, y = r % ny, x = r / ny;
+    const cplx<T> v = p[base + x * sx + y * sy + z];
+    const double re = (double)(T(s) * v.re), im = (double)(T(s) * v.im);
+    const double a2 = re * re + im * im, a = sqrt(a2);
+    mx = fmax(mx, a);
+    mn = fmin(mn, a);
+    s1 += a;
+    s2 += a2;
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = mx;
+  red[1][threadIdx.x] = mn;
+  red[2][threadIdx.x] = s1;
+  red[3][threadIdx.x] = s2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+      red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+      red[2][threadIdx.x] += red[2][threadIdx.x + o];
+      red[3][threadIdx.x] += red[3][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) part[4 * blockIdx.x + q] = red[q][0];
+  }
+}
+
+// Fixed-order sum of nb x 4 partials with one 256-thread workgroup.
+__device__ void sum_partials(const double *part, int nb, double out[4], bool minmax) {
+  __shared__ double red[4][256];
+  double v[4];
+  if (minmax) { v[0] = 0.0; v[1] = INFINITY; v[2] = 0.0; v[3] = 0.0; }
+  else { v[0] = v[1] = v[2] = v[3] = 0.0; }
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    if (minmax) {
+      v[0] = fmax(v[0], part[4 * i]);
+      v[1] = fmin(v[1], part[4 * i + 1]);
+    } else {
+      v[0] += part[4 * i];
+      v[1] += part[4 * i + 1];
+    }
+    v[2] += part[4 * i + 2];
+    v[3] += part[4 * i + 3];
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      if (minmax) {
+        red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+        red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+      } else {
+        red[0][threadIdx.x] += red[0][threadIdx.x + o];
+        red[1][threadIdx.x] += red[1][threadIdx.x + o];
+      }
+      red[2][threadIdx.x] += red[2][threadIdx.x + o];
+      red[3][threadIdx.x] += red[3][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  for (int q = 0; q < 4; ++q) out[q] = red[q][0];
+}
+
+// After a stage-1 (or rhs-only) pass with reductions: energy, mu, residual; the
+// convergence test; c0 for the next step.  st[] layout in tbec_internal.hpp.
+__global__ void __launch_bounds__(256)
+k_fin_energy(const double *part, int nb, double *st, int *flag, int check_conv) {
+  if (check_conv && flag[0]) return;
+  double s[4];
+  sum_partials(part, nb, s, false);
+  if (threadIdx.x != 0) return;
+  const double dV = st[ST_DV];
+  const double N = s[0] * dV, S1 = s[1] * dV, S4 = s[2] * dV, R = s[3] * dV;
+  const double c0 = check_conv ? st[ST_C0EFF] : st[ST_C0BASE];
+  const double H = -0.5 * S1 + 0.5 * c0 * N;
+  const double E = H - 0.25 * st[ST_GNL] * S4;
+  const double mu = H / N;
+  const double am = 0.5 * c0 - mu;
+  double r2 = 0.25 * R - am * S1 + am * am * N;
+  if (r2 < 0) r2 = 0;
+  const double res = sqrt(r2 / N);
+  st[ST_N] = N;
+  st[ST_E] = E;
+  st[ST_MU] = mu;
+  st[ST_RES] = res;
+  if (!check_conv) return;
+  if (!isfinite(E) || !isfinite(N) || N <= 0.0) {
+    flag[0] = 2;
+    flag[1] = (int)st[ST_STEP];
+    return;
+  }
+  const double rel = st[ST_STEP] > 0.0 ? fabs(E - st[ST_EPREV]) / fabs(E) : INFINITY;
+  st[ST_REL] = rel;
+  st[ST_EPREV] = E;
+  st[ST_C0NEXT] = st[ST_C0BASE] + 2.0 * mu;
+  if (rel < st[ST_TOL]) {
+    flag[0] = 1;
+    flag[1] = (int)st[ST_STEP];
+  }
+}
+
+// After stage 3 with reductions: renormalisation scale for the next read, c0 update.
+__global__ void __launch_bounds__(256)
+k_fin_norm(const double *part, int nb, double *st, int *flag) {
+  if (flag[0]) return;
+  double s[4];
+  sum_partials(part, nb, s, false);
+  if (threadIdx.x != 0) return;
+  const double N = s[0] * st[ST_DV];
+  if (!isfinite(N) || N <= 0.0) {
+    flag[0] = 2;
+    flag[1] = (int)st[ST_STEP];
+    return;
+  }
+  st[ST_SCALE] = sqrt(st[ST_NORMT] / N);
+  st[ST_C0EFF] = st[ST_C0NEXT];
+  st[ST_STEP] += 1.0;
+}
+
+__global__ void __launch_bounds__(256) k_fin_info(const double *part, int nb, double *st) {
+  double s[4];
+  sum_partials(part, nb, s, true);
+  if (threadIdx.x != 0) return;
+  st[ST_MAXA] = s[0];
+  st[ST_MINA] = s[1];
+  st[ST_SUMA] = s[2];
+  st[ST_SUMA2] = s[3];
+}
+
+}  // namespace tbec
+
+// ------------------------------------------------------------------ handle ------
+namespace tbec {
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A, double B,
+                     double cre, double cim) {
+  const TileCfg &tc = tile_cfg(h->p.prec, h->tile);
+  a.psi_in = h->psi[src];
+  a.psi_out = h->psi[src ^ 1];
+  a.w = h->w;
+  a.nx = h->p.nx; a.ny = h->p.ny; a.nz = h->p.nz;
+  a.seg_len = h->seg;
+  a.tiles_z = (int)((h->p.nz + tc.tz - 1) / tc.tz);
+  a.tiles_y = (int)((h->p.ny + tc.ty - 1) / tc.ty);
+  a.nseg = (h->p.nx + h->seg - 1) / h->seg;
+  a.nblocks = a.tiles_z * a.tiles_y * a.nseg;
+  a.sy = h->sy; a.sx = h->sx; a.base = h->base;
+  a.x0 = -0.5 * h->p.Lx + 0.5 * h->hx;
+  a.y0 = -0.5 * h->p.Ly + 0.5 * h->hy;
+  a.z0 = -0.5 * h->p.Lz + 0.5 * h->hz;
+  a.hx = h->hx; a.hy = h->hy; a.hz = h->hz;
+  a.k2[0] = 1.0 / (180.0 * h->hx * h->hx);
+  a.k2[1] = 1.0 / (180.0 * h->hy * h->hy);
+  a.k2[2] = 1.0 / (180.0 * h->hz * h->hz);
+  a.k1[0] = 1.0 / (60.0 * h->hx);
+  a.k1[1] = 1.0 / (60.0 * h->hy);
+  a.wl[0] = h->p.harm_wx / (h->p.Lx * h->p.Lx);
+  a.wl[1] = h->p.harm_wy / (h->p.Ly * h->p.Ly);
+  a.wl[2] = h->p.harm_wz / (h->p.Lz * h->p.Lz);
+  a.c0 = h->p.c0; a.gnl = h->p.gnl; a.om = h->p.omega_z;
+  a.cre = cre; a.cim = cim; a.A = A; a.B = B;
+  a.stage = stage;
+  a.scale = nullptr; a.c0dev = nullptr; a.skip = nullptr; a.partials = nullptr;
+  return a.nblocks;
+}
+
+bool handle_rot(const tbec_handle *h) {
+  // the reference enables the frame term when |omega_z| > epsilon(0.) of the working
+  // precision (src/finite_diff.f90:128)
+  const double eps = h->p.prec == TBEC_F32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+  return std::fabs(h->p.omega_z) > eps;
+}
+
+int tbec_reserve_partials(tbec_handle *h, int nb) {
+  if (nb > h->npart) {
+    if (h->partials) hipFree(h->partials);
+    h->partials = nullptr;
+    HIPCHK(hipMalloc(&h->partials, sizeof(double) * 4 * (size_t)nb));
+    h->npart = nb;
+  }
+  return 0;
+}
+
+// Enqueue one stage.  kind: StageKind; imag: use the imaginary-time device state.
+int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
+  StageArgs a;
+  const double dr = h->dt_re, di = h->dt_im;
+  // eye*0.5*dt (src/run.f90:184)
+  double cre = -0.5 * di, cim = 0.5 * dr;
+  double A = 0.0, B = 1.0;
+  if (kind == ST_RK1) { A = 0.0; B = 1.0 / 3.0; }
+  else if (kind == ST_RK2) { A = -2.0 / 3.0; B = 1.0; }
+  else if (kind == ST_RK3) { A = -1.0; B = 0.5; }
+  else if (kind == ST_EULER) { cre = 0.5 * h->p.dt_re; cim = 0.5 * h->p.dt_im; }  // dt*0.5
+  int nb = fill_args(h, a, kind, h->cur, A, B, cre, cim);
+  const bool want_red = imag || reduce;
+  if (want_red) {
+    int rc = tbec_reserve_partials(h, nb);
+    if (rc) return rc;
+    a.partials = h->partials;
+  }
+  if (imag) {
+    a.c0dev = h->dstate + ST_C0EFF;
+    a.skip = h->dflag;
+    if (kind == ST_RK1) a.scale = h->dstate + ST_SCALE;
+  } else if (reduce) {
+    a.scale = h->dstate + ST_SCALE;
+  }
+  hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, handle_rot(h), want_red,
+                              h->stream);
+  if (e != hipSuccess)
+    return fail(TBEC_ERR_HIP, "stage launch failed: %s", hipGetErrorString(e));
+  if (imag) {
+    if (kind == ST_RK1) {
+      hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb,
+                         h->dstate, h->dflag, 1);
+    } else if (kind == ST_RK3) {
+      hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(256), 0, h->stream, h->partials, nb,
+                         h->dstate, h->dflag);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  if (kind != ST_RHS) h->cur ^= 1;
+  return 0;
+}
+
+static int grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace tbec
+
+extern "C" {
+
+const char *tbec_last_error(void) { return g_err.c_str(); }
+
+const char *tbec_version(void) { return "libtbec 0.1 (gfx950)"; }
+
+int tbec_create(const tbec_params *p, tbec_handle **out) {
+  if (!p || !out) return fail(TBEC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1)
+    return fail(TBEC_ERR_ARG, "grid must be positive (got %d x %d x %d)", p->nx, p->ny, p->nz);
+  if (p->prec != TBEC_F32 && p->prec != TBEC_F64)
+    return fail(TBEC_ERR_ARG, "prec must be 32 or 64 (got %d)", p->prec);
+  if (p->sum_mode != TBEC_SUM_REFERENCE && p->sum_mode != TBEC_SUM_COMPENSATED)
+    return fail(TBEC_ERR_ARG, "unknown sum_mode %d", p->sum_mode);
+  if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0))
+    return fail(TBEC_ERR_ARG, "box lengths must be positive");
+  if (p->tile < -1 || p->tile >= kNumTiles)
+    return fail(TBEC_ERR_ARG, "tile id %d out of range [-1, %d)", p->tile, kNumTiles);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  tbec_handle *h = new tbec_handle();
+  h->p = *p;
+  h->tile = p->tile < 0 ? 0 : p->tile;
+  h->seg = 64;
+  h->csize = p->prec == TBEC_F32 ? 8 : 16;
+  h->hx = p->Lx / p->nx;
+  h->hy = p->Ly / p->ny;
+  h->hz = p->Lz / p->nz;
+  // dt*(1 - i*gamma) for real time (src/run.f90:179)
+  h->dt_re = p->dt_re + p->gamma * p->dt_im;
+  h->dt_im = p->dt_im - p->gamma * p->dt_re;
+  // padded layout: interior z starts at a 128-byte boundary; tiles of up to 128 in z and
+  // 64 in y may overrun the interior and read zeros; x has 3 ghost planes below and 4
+  // above (the register queue prefetches plane x+4).
+  const long long zoff = 128 / h->csize;
+  h->PZ = round_up(zoff + round_up(p->nz, 128) + 4, zoff);
+  h->PY = 3 + round_up(p->ny, 64) + 3;
+  h->PX = 3 + (long long)p->nx + 4;
+  h->sy = h->PZ;
+  h->sx = h->PZ * h->PY;
+  h->base = 3 * h->sx + 3 * h->sy + zoff;
+  h->elems = h->sx * h->PX;
+  const size_t bytes = (size_t)h->elems * h->csize;
+  hipError_t e1 = hipMalloc(&h->psi[0], bytes);
+  hipError_t e2 = e1 == hipSuccess ? hipMalloc(&h->psi[1], bytes) : e1;
+  hipError_t e3 = e2 == hipSuccess ? hipMalloc(&h->w, bytes) : e2;
+  if (e3 != hipSuccess) {
+    tbec_destroy(h);
+    (void)hipGetLastError();
+    return fail(TBEC_ERR_OOM, "device allocation of 3 x %.3f GB failed: %s", bytes / 1e9,
+                hipGetErrorString(e3));
+  }
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHK(hipMemsetAsync(h->psi[0], 0, bytes, h->stream));
+  HIPCHK(hipMemsetAsync(h->psi[1], 0, bytes, h->stream));
+  HIPCHK(hipMemsetAsync(h->w, 0, bytes, h->stream));
+  HIPCHK(hipMalloc(&h->dstate, sizeof(double) * ST_COUNT));
+  HIPCHK(hipMalloc(&h->dflag, sizeof(int) * 4));
+  HIPCHK(hipHostMalloc(&h->hstate, sizeof(double) * ST_COUNT));
+  HIPCHK(hipHostMalloc(&h->hflag, sizeof(int) * 4));
+  h->norm_target = 1.0;
+  int rc = tbec_reset_state(h);
+  if (rc) { tbec_destroy(h); return rc; }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *out = h;
+  return 0;
+}
+
+int tbec_destroy(tbec_handle *h) {
+  if (!h) return 0;
+  tbec_drop_graphs(h);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->psi[0]) hipFree(h->psi[0]);
+  if (h->psi[1]) hipFree(h->psi[1]);
+  if (h->w) hipFree(h->w);
+  if (h->stage) hipFree(h->stage);
+  if (h->partials) hipFree(h->partials);
+  if (h->dstate) hipFree(h->dstate);
+  if (h->dflag) hipFree(h->dflag);
+  if (h->hstate) hipHostFree(h->hstate);
+  if (h->hflag) hipHostFree(h->hflag);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int tbec_set_norm(tbec_handle *h, double norm) {
+  if (!h || !(norm > 0)) return fail(TBEC_ERR_ARG, "norm must be positive");
+  h->norm_target = norm;
+  return tbec_reset_state(h);
+}
+
+static int ensure_stage_buf(tbec_handle *h) {
+  if (!h->stage) {
+    const size_t n = (size_t)h->p.nx * h->p.ny * h->p.nz * h->csize;
+    hipError_t e = hipMalloc(&h->stage, n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(TBEC_ERR_OOM, "staging buffer allocation failed: %s", hipGetErrorString(e));
+    }
+  }
+  return 0;
+}
+
+static int unpack_from(tbec_handle *h, const void *src_dev) {
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_unpack<float>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (const cplx<float> *)src_dev, (cplx<float> *)h->psi[h->cur], h->p.nx,
+                       h->p.ny, h->p.nz, h->sy, h->sx, h->base);
+  else
+    hipLaunchKernelGGL(k_unpack<double>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (const cplx<double> *)src_dev, (cplx<double> *)h->psi[h->cur], h->p.nx,
+                       h->p.ny, h->p.nz, h->sy, h->sx, h->base);
+  HIPCHK(hipGetLastError());
+  return tbec_reset_state(h);
+}
+
+static int pack_to(tbec_handle *h, const void *src_padded, void *dst_dev, const double *scale) {
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (const cplx<float> *)src_padded, (cplx<float> *)dst_dev, h->p.nx, h->p.ny,
+                       h->p.nz, h->sy, h->sx, h->base, scale);
+  else
+    hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (const cplx<double> *)src_padded, (cplx<double> *)dst_dev, h->p.nx,
+                       h->p.ny, h->p.nz, h->sy, h->sx, h->base, scale);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int tbec_upload_psi(tbec_handle *h, const void *psi_host) {
+  if (!h || !psi_host) return fail(TBEC_ERR_ARG, "null argument");
+  int rc = ensure_stage_buf(h);
+  if (rc) return rc;
+  const size_t n = (size_t)h->p.nx * h->p.ny * h->p.nz * h->csize;
+  HIPCHK(hipMemcpyAsync(h->stage, psi_host, n, hipMemcpyHostToDevice, h->stream));
+  rc = unpack_from(h, h->stage);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int tbec_upload_psi_device(tbec_handle *h, const void *psi_dev) {
+  if (!h || !psi_dev) return fail(TBEC_ERR_ARG, "null argument");
+  return unpack_from(h, psi_dev);
+}
+
+int tbec_download_psi(tbec_handle *h, void *psi_host) {
+  if (!h || !psi_host) return fail(TBEC_ERR_ARG, "null argument");
+  int rc = ensure_stage_buf(h);
+  if (rc) return rc;
+  rc = pack_to(h, h->psi[h->cur], h->stage, h->dstate + ST_SCALE);
+  if (rc) return rc;
+  const size_t n = (size_t)h->p.nx * h->p.ny * h->p.nz * h->csize;
+  HIPCHK(hipMemcpyAsync(psi_host, h->stage, n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int tbec_download_psi_device(tbec_handle *h, void *psi_dev) {
+  if (!h || !psi_dev) return fail(TBEC_ERR_ARG, "null argument");
+  return pack_to(h, h->psi[h->cur], psi_dev, h->dstate + ST_SCALE);
+}
+
+int tbec_rhs(tbec_handle *h, void *rhs_host) {
+  if (!h || !rhs_host) return fail(TBEC_ERR_ARG, "null argument");
+  int rc = tbec_materialize_scale(h);
+  if (rc) return rc;
+  rc = enqueue_stage(h, ST_RHS, false, false);
+  if (rc) return rc;
+  rc = ensure_stage_buf(h);
+  if (rc) return rc;
+  rc = pack_to(h, h->w, h->stage, nullptr);
+  if (rc) return rc;
+  const size_t n = (size_t)h->p.nx * h->p.ny * h->p.nz * h->csize;
+  HIPCHK(hipMemcpyAsync(rhs_host, h->stage, n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int tbec_diagnose(tbec_handle *h, tbec_diag *out) {
+  if (!h || !out) return fail(TBEC_ERR_ARG, "null argument");
+  // rhs-only pass with reductions (psi read with the pending scale), then Psi_info
+  int rc = enqueue_stage(h, ST_RHS, false, true);
+  if (rc) return rc;
+  const int nb_rhs = fill_args_count(h);
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,
+                     h->dstate, h->dflag, 0);
+  HIPCHK(hipGetLastError());
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  const int nbi = grid_for(n);
+  rc = tbec_reserve_partials(h, nbi);
+  if (rc) return rc;
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_info<float>, dim3(nbi), dim3(256), 0, h->stream,
+                       (const cplx<float> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
+                       h->sx, h->base, h->dstate + ST_SCALE, h->partials);
+  else
+    hipLaunchKernelGGL(k_info<double>, dim3(nbi), dim3(256), 0, h->stream,
+                       (const cplx<double> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
+                       h->sx, h->base, h->dstate + ST_SCALE, h->partials);
+  hipLaunchKernelGGL(k_fin_info, dim3(1), dim3(256), 0, h->stream, h->partials, nbi,
+                     h->dstate);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->hstate, h->dstate, sizeof(double) * ST_COUNT,
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double *s = h->hstate;
+  out->norm = s[ST_N];
+  out->energy = s[ST_E];
+  out->mu = s[ST_MU];
+  out->residual = s[ST_RES];
+  out->max_abs = s[ST_MAXA];
+  out->min_abs = s[ST_MINA];
+  out->mean_abs = s[ST_SUMA] / ((double)h->p.nx * h->p.ny * h->p.nz);
+  out->int_psi2 = s[ST_SUMA2];
+  if (!std::isfinite(out->energy) || !std::isfinite(out->norm))
+    return fail(TBEC_ERR_NONFINITE, "non-finite norm or energy in diagnostics");
+  return 0;
+}
+
+int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (tile < 0 || tile >= kNumTiles) return fail(TBEC_ERR_ARG, "tile id %d out of range", tile);
+  if (seg < 1 || seg > 4096) return fail(TBEC_ERR_ARG, "segment length %d out of range", seg);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  tbec_drop_graphs(h);
+  h->tile = tile;
+  h->seg = seg;
+  return 0;
+}
+
+int tbec_sync(tbec_handle *h) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+void *tbec_stream(tbec_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+}  // extern "C"
+
+// ------------------------------------------------------ state helpers -----------
+int fill_args_count(tbec_handle *h) {
+  StageArgs a;
+  return fill_args(h, a, ST_RHS, h->cur, 0, 0, 0, 0);
+}
+
+int tbec_reset_state(tbec_handle *h) {
+  double *s = h->hstate;
+  for (int i = 0; i < ST_COUNT; ++i) s[i] = 0.0;
+  s[ST_SCALE] = 1.0;
+  s[ST_C0EFF] = s[ST_C0NEXT] = s[ST_C0BASE] = h->p.c0;
+  s[ST_GNL] = h->p.gnl;
+  s[ST_DV] = h->hx * h->hy * h->hz;
+  s[ST_NORMT] = h->norm_target;
+  s[ST_TOL] = h->tol;
+  HIPCHK(hipMemcpyAsync(h->dstate, s, sizeof(double) * ST_COUNT, hipMemcpyHostToDevice,
+                        h->stream));
+  for (int i = 0; i < 4; ++i) h->hflag[i] = 0;
+  HIPCHK(hipMemcpyAsync(h->dflag, h->hflag, sizeof(int) * 4, hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->scale_pending = false;
+  return 0;
+}
+
+// Fold the pending renormalisation scale into the stored psi (scale*psi rounded to the
+// working precision, exactly what the next stage-1 read would compute), scale = 1.
+int tbec_materialize_scale(tbec_handle *h) {
+  if (!h->scale_pending) return 0;
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_apply_scale<float>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (cplx<float> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy, h->sx,
+                       h->base, (const double *)(h->dstate + ST_SCALE));
+  else
+    hipLaunchKernelGGL(k_apply_scale<double>, dim3(grid_for(n)), dim3(256), 0, h->stream,
+                       (cplx<double> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
+                       h->sx, h->base, (const double *)(h->dstate + ST_SCALE));
+  hipLaunchKernelGGL(k_reset_scale, dim3(1), dim3(1), 0, h->stream, h->dstate);
+  HIPCHK(hipGetLastError());
+  h->scale_pending = false;
+  return 0;
+}

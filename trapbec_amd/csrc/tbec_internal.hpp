@@ -1,0 +1,69 @@
+// Internal state of a libtbec handle (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+
+#include "tbec.h"
+
+namespace tbec {
+
+// Device state slots (doubles) of a handle.
+enum StateSlot {
+  ST_SCALE = 0,   // psi is stored unnormalised; stage 1 reads scale*psi
+  ST_C0EFF,       // c0 used by the imaginary-time stages of the current step
+  ST_C0NEXT,      // c0 + 2*mu of the current step, for the next one
+  ST_C0BASE,      // c0 of the parameters
+  ST_GNL,
+  ST_DV,
+  ST_NORMT,       // renormalisation target
+  ST_TOL,         // relative energy change that stops the solve
+  ST_STEP,        // completed imaginary-time steps
+  ST_EPREV,
+  ST_N, ST_E, ST_MU, ST_RES, ST_REL,
+  ST_MAXA, ST_MINA, ST_SUMA, ST_SUMA2,
+  ST_COUNT
+};
+
+struct TileCfg { int tz, ty, vz; };
+const TileCfg &tile_cfg(int prec, int id);
+extern const int kNumTiles;
+
+}  // namespace tbec
+
+struct tbec_handle {
+  tbec_params p{};
+  int tile = 0, seg = 64;
+  int csize = 8;
+  double hx = 0, hy = 0, hz = 0;
+  double dt_re = 0, dt_im = 0;          // dt*(1 - i*gamma)
+  long long PZ = 0, PY = 0, PX = 0, sy = 0, sx = 0, base = 0, elems = 0;
+  void *psi[2] = {nullptr, nullptr};
+  void *w = nullptr;
+  void *stage = nullptr;                // interior-sized staging buffer (lazy)
+  int cur = 0;                          // psi[cur] holds the current state
+  double *partials = nullptr;
+  int npart = 0;
+  double *dstate = nullptr, *hstate = nullptr;
+  int *dflag = nullptr, *hflag = nullptr;
+  double norm_target = 1.0;
+  double tol = 0.0;
+  bool scale_pending = false;
+  hipStream_t stream = nullptr;
+  // captured step graphs: (mode, steps, parity) -> executable graph
+  std::map<std::tuple<int, long long, int>, hipGraphExec_t> graphs;
+};
+
+namespace tbec {
+int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce);
+int tbec_reserve_partials(tbec_handle *h, int nb);
+bool handle_rot(const tbec_handle *h);
+int set_error(int code, const std::string &msg);
+}  // namespace tbec
+
+int tbec_reset_state(tbec_handle *h);
+int tbec_materialize_scale(tbec_handle *h);
+void tbec_drop_graphs(tbec_handle *h);
+int fill_args_count(tbec_handle *h);

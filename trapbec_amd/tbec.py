@@ -1,0 +1,201 @@
+"""Python host interface to libtbec (ctypes over the C ABI in include/tbec.h).
+
+This mirrors the reference's driver role (src/run.f90): parameters as in cparam.local
+(src/cparam.local:1-44), psi in/out as in reload_data / var_print
+(src/initial.f90:796-818, src/output.f90:71-108), the real-time RK3 loop
+(src/run.f90:179-192), the imaginary-time loop (src/run.f90:156-170) and the
+diagnostics (src/diagnostic.f90:7-25).  All compute runs in libtbec's HIP kernels;
+there is no CPU fallback -- a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtbec.so")
+
+TBEC_F32, TBEC_F64 = 32, 64
+SUM_REFERENCE, SUM_COMPENSATED = 0, 1
+REAL_RK3, IMAG_RK3, IMAG_EULER_REF = 0, 1, 2
+ERRORS = {-1: "TBEC_ERR_ARG", -2: "TBEC_ERR_OOM", -3: "TBEC_ERR_HIP",
+          -4: "TBEC_ERR_NONFINITE", -5: "TBEC_ERR_STATE"}
+
+
+class TbecParams(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("prec", C.c_int32), ("sum_mode", C.c_int32), ("tile", C.c_int32),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("harm_wx", C.c_double), ("harm_wy", C.c_double), ("harm_wz", C.c_double),
+                ("omega_z", C.c_double), ("c0", C.c_double), ("gnl", C.c_double),
+                ("dt_re", C.c_double), ("dt_im", C.c_double), ("gamma", C.c_double)]
+
+
+class TbecDiag(C.Structure):
+    _fields_ = [("norm", C.c_double), ("energy", C.c_double), ("mu", C.c_double),
+                ("residual", C.c_double), ("max_abs", C.c_double), ("min_abs", C.c_double),
+                ("mean_abs", C.c_double), ("int_psi2", C.c_double)]
+
+
+class TbecSolveResult(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("converged", C.c_int32), ("pad_", C.c_int32),
+                ("energy", C.c_double), ("mu", C.c_double), ("residual", C.c_double),
+                ("norm", C.c_double), ("rel_change", C.c_double)]
+
+
+class TbecError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi",
+           "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
+           "tbec_diagnose", "tbec_imag_solve", "tbec_set_norm", "tbec_sync", "tbec_stream",
+           "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libtbec.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    H = C.c_void_p
+    vp = C.c_void_p
+    L.tbec_create.argtypes = [C.POINTER(TbecParams), C.POINTER(H)]
+    L.tbec_destroy.argtypes = [H]
+    for f in ("tbec_upload_psi", "tbec_download_psi", "tbec_upload_psi_device",
+              "tbec_download_psi_device", "tbec_rhs"):
+        getattr(L, f).argtypes = [H, vp]
+    L.tbec_step.argtypes = [H, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.tbec_diagnose.argtypes = [H, C.POINTER(TbecDiag)]
+    L.tbec_imag_solve.argtypes = [H, C.c_int64, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(TbecSolveResult)]
+    L.tbec_set_norm.argtypes = [H, C.c_double]
+    L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
+    L.tbec_sync.argtypes = [H]
+    L.tbec_stream.argtypes = [H]
+    L.tbec_stream.restype = C.c_void_p
+    L.tbec_time_stages.argtypes = [H, C.c_int, C.POINTER(C.c_double)]
+    L.tbec_last_error.restype = C.c_char_p
+    L.tbec_version.restype = C.c_char_p
+    for f in EXPORTS:
+        if f not in ("tbec_stream", "tbec_last_error", "tbec_version"):
+            getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise TbecError(rc, load().tbec_last_error().decode())
+
+
+class Condensate:
+    """One condensate on the GPU (a libtbec handle).
+
+    psi arrays are numpy complex64/complex128 (or torch device tensors for the *_device
+    calls) indexed [x, y, z], C order -- the reference's Psi(k, j, i) memory order."""
+
+    def __init__(self, shape, box, harm_w, *, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3,
+                 gamma=0.0, prec=64, sum_mode=None, tile=-1):
+        L = load()
+        if sum_mode is None:
+            sum_mode = SUM_REFERENCE if prec == 64 else SUM_COMPENSATED
+        box = (box, box, box) if np.isscalar(box) else tuple(box)
+        dt = complex(dt)
+        self.shape, self.box, self.prec = tuple(shape), box, prec
+        self.cdt = np.complex128 if prec == 64 else np.complex64
+        self.params = TbecParams(shape[0], shape[1], shape[2], prec, sum_mode, tile,
+                                 box[0], box[1], box[2], harm_w[0], harm_w[1], harm_w[2],
+                                 omega_z, c0, gnl, dt.real, dt.imag, gamma)
+        self._h = C.c_void_p()
+        _check(L.tbec_create(C.byref(self.params), C.byref(self._h)))
+        self.t = 0.0
+
+    @classmethod
+    def from_problem(cls, pr, prec=64, **kw):
+        kp = pr.kernel_params()
+        return cls(pr.shape, pr.box, kp["harm_w"], omega_z=kp["omega_r"], c0=kp["c0"],
+                   gnl=kp["gnl"], dt=kp["dt"], prec=prec, **kw)
+
+    def close(self):
+        if self._h:
+            load().tbec_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # data movement
+    def upload(self, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=self.cdt)
+        assert a.shape == self.shape
+        _check(load().tbec_upload_psi(self._h, a.ctypes.data_as(C.c_void_p)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.cdt)
+        _check(load().tbec_download_psi(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def upload_device(self, ptr: int):
+        _check(load().tbec_upload_psi_device(self._h, C.c_void_p(ptr)))
+
+    def download_device(self, ptr: int):
+        _check(load().tbec_download_psi_device(self._h, C.c_void_p(ptr)))
+
+    # stepping
+    def step(self, nsteps: int, mode: int = REAL_RK3):
+        t = C.c_double(self.t)
+        _check(load().tbec_step(self._h, nsteps, mode, C.byref(t)))
+        self.t = t.value
+
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.cdt)
+        _check(load().tbec_rhs(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def diagnose(self) -> dict:
+        d = TbecDiag()
+        _check(load().tbec_diagnose(self._h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in TbecDiag._fields_}
+
+    def imag_solve(self, max_steps: int, tol: float, norm: float = 1.0,
+                   check_every: int = 50) -> dict:
+        r = TbecSolveResult()
+        _check(load().tbec_imag_solve(self._h, max_steps, tol, norm, check_every,
+                                      C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TbecSolveResult._fields_ if k != "pad_"}
+
+    def set_norm(self, norm: float):
+        _check(load().tbec_set_norm(self._h, norm))
+
+    def set_tuning(self, tile: int, seg: int):
+        _check(load().tbec_set_tuning(self._h, tile, seg))
+
+    def sync(self):
+        _check(load().tbec_sync(self._h))
+
+    def stream_ptr(self) -> int:
+        return load().tbec_stream(self._h)
+
+    def time_stages(self, nstages: int) -> float:
+        ms = C.c_double()
+        _check(load().tbec_time_stages(self._h, nstages, C.byref(ms)))
+        return ms.value
