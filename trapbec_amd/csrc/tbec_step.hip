@@ -52,6 +52,9 @@ static int replay_chunk(tbec_handle *h, long long n, int mode) {
   auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {
     const int cur0 = h->cur;
+    // no allocation may happen inside the capture: size the reduction buffer first
+    int rc0 = tbec_reserve_partials(h, fill_args_count(h));
+    if (rc0) return rc0;
     HIPCHK2(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_steps(h, n, mode);
     hipGraph_t g = nullptr;

@@ -1,0 +1,274 @@
+! tbec_run: the Fortran host program for libtbec.
+!
+! It keeps the reference's driver role (src/run.f90:1-81): run parameters (here a
+! runtime namelist instead of the compile-time src/cparam.local), the initial state
+! (src/initial.f90), the imaginary-time stage (src/run.f90:156-172), the real-time loop
+! with its output cadence (src/run.f90:179-213), the per-shot diagnostics line
+! (src/run.f90:197-200), the var files and restart file (src/output.f90:71-108) and the
+! timing log (src/output.f90:111-118).  Every step runs on the GPU through libtbec.
+!
+!   tbec_run [namelist file]       (default: tbec.nml)
+!
+! Errors follow the reference's emergency_stop (src/mpi_var.f90:45-62): message on
+! stdout, the text in ./ERROR, non-zero exit.
+program tbec_run
+  use iso_c_binding
+  use iso_fortran_env, only: real32, real64, int64
+  use tbec_iface
+  use tbec_ic
+  implicit none
+
+  ! ---- namelist (reference names where the reference has the parameter) ----
+  integer :: Nx = 64, Ny = 64, Nz = 64
+  real(dp) :: Lx = 16.0_dp, Ly = 16.0_dp, Lz = 16.0_dp
+  real(dp) :: dt_re = 1.0e-3_dp, dt_im = 0.0_dp, gamma = 0.0_dp
+  integer :: nsteps = 1000, shots = 1000
+  real(dp) :: harm_wx = 256.0_dp, harm_wy = 256.0_dp, harm_wz = 256.0_dp
+  real(dp) :: omega_z = 0.0_dp, c0 = 1.0_dp, gnl = 1.0_dp
+  integer :: prec = 64, sum_mode = -1, tile = -1
+  character(len=32) :: init_cond = 'tf_vortex'
+  real(dp) :: trap_omega(3) = [1.0_dp, 1.0_dp, 1.0_dp]
+  real(dp) :: mu = 10.0_dp, g_phys = 500.0_dp, tf_norm = 0.0_dp, psi_scale = 1.0_dp
+  integer :: n_lines = 1
+  real(dp) :: line_x(8) = 0.0_dp, line_y(8) = 0.0_dp
+  integer :: line_q(8) = 1
+  real(dp) :: xi = 0.0_dp, ring_radius = 0.0_dp
+  integer(int64) :: seed = 20261203_int64
+  real(dp) :: noise_amp = 0.0_dp, phase_noise_amp = 0.0_dp
+  logical :: imaginary_time = .false., ic_only = .false., full_binary_output = .true.
+  integer :: imag_nsteps = 0, imag_check = 50
+  real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
+  character(len=256) :: out_dir = 'data'
+  namelist /tbec_run/ Nx, Ny, Nz, Lx, Ly, Lz, dt_re, dt_im, gamma, nsteps, shots, &
+    harm_wx, harm_wy, harm_wz, omega_z, c0, gnl, prec, sum_mode, tile, init_cond, &
+    trap_omega, mu, g_phys, tf_norm, psi_scale, n_lines, line_x, line_y, line_q, xi, &
+    ring_radius, seed, noise_amp, phase_noise_amp, imaginary_time, ic_only, &
+    full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir
+
+  character(len=512) :: nml_file
+  integer :: u, ios, itime, nstart, l
+  real(dp), allocatable :: xx(:), yy(:), zz(:)
+  complex(dp), allocatable, target :: psi(:, :, :)
+  complex(real32), allocatable, target :: psi4(:, :, :)
+  type(tbec_params) :: p
+  type(tbec_diag) :: d
+  type(tbec_solve_result) :: sr
+  type(c_ptr) :: h
+  real(c_double) :: t
+  integer(int64) :: c_start, c_end, c_rate
+  real(dp) :: time_per_step
+
+  nml_file = 'tbec.nml'
+  if (command_argument_count() >= 1) call get_command_argument(1, nml_file)
+  open(newunit=u, file=trim(nml_file), status='old', action='read', iostat=ios)
+  if (ios /= 0) call emergency_stop('cannot open namelist file '//trim(nml_file))
+  read(u, nml=tbec_run, iostat=ios)
+  if (ios /= 0) call emergency_stop('bad namelist in '//trim(nml_file))
+  close(u)
+  if (prec /= 32 .and. prec /= 64) call emergency_stop('prec must be 32 or 64')
+  if (shots < 1) call emergency_stop('shots must be positive')
+  if (sum_mode < 0) then
+    sum_mode = TBEC_SUM_REFERENCE
+    if (prec == 32) sum_mode = TBEC_SUM_COMPENSATED
+  end if
+
+  write(*, '(a)') ' TrapBEC (MI355X) driver initialised'
+  write(*, '(a,i4.4,a,i4.4,a,i4.4,a)') ' Running a ', Nx, 'x', Ny, 'x', Nz, ' GPE simulation'
+  write(*, '(a,i2,a,i1)') ' precision: fp', prec, '  stencil sum mode: ', sum_mode
+  ! the reference's stability warning (src/initial.f90:29-33)
+  if (dt_re > (1.0_dp / 8.0_dp) * min(Lx / Nx, Ly / Ny, Lz / Nz)**2) then
+    write(*, '(a)') ' WARNING TIMESTEP IS PROBABLY TOO LARGE!'
+  end if
+
+  allocate(xx(Nx), yy(Ny), zz(Nz), psi(Nz, Ny, Nx))
+  call coords(Nx, Lx, xx)
+  call coords(Ny, Ly, yy)
+  call coords(Nz, Lz, zz)
+  t = 0.0_c_double
+  nstart = 1
+
+  select case (trim(init_cond))
+  case ('tf_vortex')
+    if (tf_norm > 0.0_dp) mu = tf_mu_for_norm(xx, yy, zz, trap_omega, g_phys, &
+                                              (Lx / Nx) * (Ly / Ny) * (Lz / Nz), tf_norm)
+    call tf_profile(psi, xx, yy, zz, trap_omega, g_phys, mu)
+    if (xi <= 0.0_dp) xi = 1.0_dp / sqrt(mu)
+    if (ring_radius > 0.0_dp) call imprint_ring(psi, xx, yy, zz, ring_radius, 0.0_dp, xi)
+    do l = 1, min(n_lines, 8)
+      call imprint_line(psi, xx, yy, line_x(l), line_y(l), line_q(l), xi)
+    end do
+    if (noise_amp > 0.0_dp) call add_noise(psi, seed, noise_amp, 0_int64)
+    if (phase_noise_amp > 0.0_dp) call add_phase_noise(psi, seed, phase_noise_amp, 0_int64)
+    psi = psi * psi_scale
+  case ('restart')
+    call read_restart()
+  case default
+    call emergency_stop('wrong initial conditions set')
+  end select
+
+  call execute_command_line('mkdir -p '//trim(out_dir)//'/proc000')
+  if (ic_only) then
+    itime = 0
+    call write_var(0)
+    write(*, '(a)') ' initial state written; ic_only set, stopping'
+    stop
+  end if
+
+  p%nx = Nx; p%ny = Ny; p%nz = Nz
+  p%prec = prec; p%sum_mode = sum_mode; p%tile = tile
+  p%Lx = Lx; p%Ly = Ly; p%Lz = Lz
+  p%harm_wx = harm_wx; p%harm_wy = harm_wy; p%harm_wz = harm_wz
+  p%omega_z = omega_z; p%c0 = c0; p%gnl = gnl
+  p%dt_re = dt_re; p%dt_im = dt_im; p%gamma = gamma
+  call check(tbec_create(p, h))
+  call upload()
+
+  if (imaginary_time .and. nstart == 1) then
+    ! the imaginary-time stage needs dt = (0, -dtau): run it on a second handle
+    call imaginary_stage()
+  end if
+
+  call system_clock(c_start, c_rate)
+  itime = nstart - 1
+  do while (itime < nsteps)
+    l = min(shots - mod(itime, shots), nsteps - itime)
+    call check(tbec_step(h, int(l, c_int64_t), TBEC_REAL_RK3, t))
+    itime = itime + l
+    if (mod(itime, shots) == 0) then
+      call check(tbec_diagnose(h, d))
+      print *, itime / shots, t, d%max_abs, d%min_abs, d%mean_abs, d%int_psi2
+      call download()
+      call write_var(itime / shots)
+    end if
+  end do
+  call check(tbec_sync(h))
+  call system_clock(c_end)
+  time_per_step = real(c_end - c_start, dp) / real(c_rate, dp) / max(1, nsteps - nstart + 1)
+  open(newunit=u, file=trim(out_dir)//'/timing.log')
+  write(u, *) int(Nx, int64) * Ny * Nz, 1, time_per_step
+  close(u)
+  call check(tbec_destroy(h))
+
+contains
+
+  subroutine emergency_stop(msg)
+    character(*), intent(in) :: msg
+    integer :: ue
+    print *, '*****FATAL ERROR ABORTING CODE*****'
+    print *, msg
+    open(newunit=ue, file='ERROR')
+    write(ue, *) msg
+    close(ue)
+    error stop 1
+  end subroutine
+
+  subroutine check(rc)
+    integer(c_int), intent(in) :: rc
+    if (rc /= 0) call emergency_stop('libtbec: '//tbec_last_error())
+  end subroutine
+
+  subroutine upload()
+    if (prec == 64) then
+      call check(tbec_upload_psi(h, c_loc(psi)))
+    else
+      psi4 = cmplx(psi, kind=real32)
+      call check(tbec_upload_psi(h, c_loc(psi4)))
+    end if
+  end subroutine
+
+  subroutine download()
+    if (prec == 64) then
+      call check(tbec_download_psi(h, c_loc(psi)))
+    else
+      if (.not. allocated(psi4)) allocate(psi4(Nz, Ny, Nx))
+      call check(tbec_download_psi(h, c_loc(psi4)))
+      psi = cmplx(psi4, kind=dp)
+    end if
+  end subroutine
+
+  subroutine imaginary_stage()
+    type(tbec_params) :: q
+    type(c_ptr) :: hi
+    q = p
+    q%dt_re = 0.0_c_double
+    q%dt_im = -abs(dt_im)
+    if (dt_im == 0.0_dp) q%dt_im = -abs(dt_re)
+    call check(tbec_create(q, hi))
+    if (prec == 64) then
+      call check(tbec_upload_psi(hi, c_loc(psi)))
+    else
+      call check(tbec_upload_psi(hi, c_loc(psi4)))
+    end if
+    call check(tbec_imag_solve(hi, int(imag_nsteps, c_int64_t), imag_tol, imag_norm, &
+                               imag_check, sr))
+    write(*, '(a,i8,a,i2,a,es22.14,a,es12.4)') ' imaginary time: steps ', sr%steps, &
+      ' converged ', sr%converged, ' E ', sr%energy, ' residual ', sr%residual
+    if (prec == 64) then
+      call check(tbec_download_psi(hi, c_loc(psi)))
+    else
+      call check(tbec_download_psi(hi, c_loc(psi4)))
+      psi = cmplx(psi4, kind=dp)
+    end if
+    call check(tbec_destroy(hi))
+    call upload()
+    write(*, '(a)') ' finished imaginary time propagation'
+  end subroutine
+
+  ! var%04d.dat: one record coords(2), Lx, Ly, Lz, t, Psi; var.dat: [itime+1] [t] [Psi]
+  ! (src/output.f90:75-77,103-107), written in the run's precision.
+  subroutine write_var(shot)
+    integer, intent(in) :: shot
+    character(len=512) :: fname
+    integer :: uv
+    integer(4) :: crd(2)
+    crd = 0
+    if (full_binary_output) then
+      write(fname, '(a,a,i4.4,a)') trim(out_dir), '/proc000/var', shot, '.dat'
+      open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
+      if (prec == 64) then
+        write(uv) crd, Lx, Ly, Lz, real(t, dp), psi
+      else
+        write(uv) crd, real(Lx, real32), real(Ly, real32), real(Lz, real32), &
+          real(t, real32), cmplx(psi, kind=real32)
+      end if
+      close(uv)
+    end if
+    write(fname, '(a,a)') trim(out_dir), '/proc000/var.dat'
+    open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
+    write(uv) int(itime + 1, 4)
+    if (prec == 64) then
+      write(uv) real(t, dp)
+      write(uv) psi
+    else
+      write(uv) real(t, real32)
+      write(uv) cmplx(psi, kind=real32)
+    end if
+    close(uv)
+  end subroutine
+
+  ! restart from out_dir/proc000/var.dat (src/initial.f90:796-818)
+  subroutine read_restart()
+    character(len=512) :: fname
+    integer :: uv, it4
+    real(dp) :: t8
+    real(real32) :: t4
+    write(fname, '(a,a)') trim(out_dir), '/proc000/var.dat'
+    open(newunit=uv, file=trim(fname), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) call emergency_stop('cannot open restart file '//trim(fname))
+    read(uv) it4
+    if (prec == 64) then
+      read(uv) t8
+      read(uv) psi
+      t = t8
+    else
+      allocate(psi4(Nz, Ny, Nx))
+      read(uv) t4
+      read(uv) psi4
+      psi = cmplx(psi4, kind=dp)
+      t = t4
+    end if
+    close(uv)
+    nstart = it4
+    write(*, '(a,es14.6)') ' succsesful restart, t=', t
+  end subroutine
+end program
