@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--seg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    ap.add_argument("--fp32-sum", choices=["compensated", "centred", "reference"],
+                    default="compensated")
+    ap.add_argument("--variants", action="store_true",
+                    help="also time the other FP32 summation modes")
     return ap.parse_args()
 
 
@@ -63,10 +67,14 @@ def time_steps(torch, c, steps, warmup):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def run_gpu(args, prec, steps, psi_dev, pr):
+SUMS = {"reference": 0, "compensated": 1, "centred": 2}
+
+
+def run_gpu(args, prec, steps, psi_dev, pr, sum_name=None):
     import torch
     from trapbec_amd import tbec, autotune
-    c = tbec.Condensate.from_problem(pr, prec=prec)
+    sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
+    c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
     tile, seg = autotune.lookup(pr.shape, prec)
     if args.tile >= 0:
         tile = args.tile
@@ -161,6 +169,14 @@ def main():
         res[prec] = run_gpu(args, prec, steps, psi_dev, pr)
         print(f"[bench] fp{prec}: {res[prec]['ms_per_step']:.3f} ms/step "
               f"{res[prec]['gpt_steps_per_s']:.2f} Gpt-steps/s", file=sys.stderr, flush=True)
+        if prec == 32 and args.variants:
+            for name in SUMS:
+                if name == args.fp32_sum:
+                    continue
+                v = run_gpu(args, 32, steps, psi_dev, pr, name)
+                res.setdefault("variants", {})[name] = v
+                print(f"[bench] fp32 {name}: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
+                      flush=True)
         del psi_dev
         torch.cuda.empty_cache()
     head = 32 if 32 in res else 64
@@ -181,7 +197,7 @@ def main():
         "config": {"workload": f"real-time GPE RK3, {n}^3, box [-16,16)^3, g=500, dt=5e-4, "
                                f"zero ghosts, Omega=0",
                    "grid": [n, n, n], "stencil": "6th-order FD, fused RK3 stage",
-                   "fp32_sum": "compensated", "tile": r["tile"], "seg": r["seg"]},
+                   "fp32_sum": args.fp32_sum, "tile": r["tile"], "seg": r["seg"]},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
@@ -197,6 +213,10 @@ def main():
                        "frac": round(f["achieved_Bps"] / HBM_PEAK, 4),
                        "achieved_GBps": round(f["achieved_Bps"] / 1e9, 1),
                        "steps": args.fp64_steps}
+    for name, v in res.get("variants", {}).items():
+        out.setdefault("fp32_variants", {})[name] = {
+            "value": round(v["gpt_steps_per_s"], 4), "ms_per_step": round(v["ms_per_step"], 5),
+            "frac": round(v["achieved_Bps"] / HBM_PEAK, 4)}
     traffic = os.path.join(ROOT, "profiles", f"traffic_m{n}.json")
     if os.path.exists(traffic):
         with open(traffic) as fh:

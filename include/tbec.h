@@ -32,10 +32,14 @@ enum {
 
 enum { TBEC_F32 = 32, TBEC_F64 = 64 };
 
-/* Stencil summation: TBEC_SUM_REFERENCE keeps the reference's left-to-right 7-tap
- * sum (src/deriv.f90:24); TBEC_SUM_COMPENSATED forms each symmetric pair with an
- * error-free TwoSum and differences it against 2*f0 before weighting. */
-enum { TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1 };
+/* Stencil summation (FP32 only; FP64 always uses the reference order):
+ *   TBEC_SUM_REFERENCE   the reference's left-to-right 7-tap sum (src/deriv.f90:24)
+ *   TBEC_SUM_COMPENSATED each symmetric pair f[-k]+f[k] by an error-free TwoSum (s, e),
+ *                        d_k = (s - 2 f0) + e, then 270 d1 - 27 d2 + 2 d3
+ *   TBEC_SUM_CENTRED     d_k = (f[-k] - f0) + (f[k] - f0), both differences exact
+ *                        (Sterbenz) for smooth data; same rounding as COMPENSATED,
+ *                        fewer instructions. */
+enum { TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1, TBEC_SUM_CENTRED = 2 };
 
 /* Step modes. */
 enum {

@@ -1,0 +1,125 @@
+"""The Fortran host driver (trapbec_amd/driver): namelist, seeded initial state, var and
+restart files in the reference's record layout (src/output.f90:71-108), restart
+(src/initial.f90:796-818) and errors in the emergency_stop style."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle
+from trapbec_amd import initial
+
+DRV = os.path.join(ROOT, "trapbec_amd", "driver", "bin", "tbec_run")
+
+
+def _need_driver():
+    if not os.path.exists(DRV):
+        pytest.skip("driver not built")
+
+
+def _nml(path, **kw):
+    lines = ["&tbec_run"]
+    for k, v in kw.items():
+        if isinstance(v, bool):
+            v = ".true." if v else ".false."
+        elif isinstance(v, str):
+            v = f"'{v}'"
+        elif isinstance(v, float):
+            v = repr(v).replace("e", "d") if "e" in repr(v) else repr(v) + "d0"
+        lines.append(f"  {k}={v},")
+    lines.append("/")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _read_records(path):
+    out = []
+    data = open(path, "rb").read()
+    pos = 0
+    while pos < len(data):
+        n = int(np.frombuffer(data[pos:pos + 4], np.int32)[0])
+        out.append(data[pos + 4:pos + 4 + n])
+        pos += 8 + n
+    return out
+
+
+def _read_var(path, shape, prec=64):
+    rdt = np.float64 if prec == 64 else np.float32
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rec = _read_records(path)[0]
+    coords = np.frombuffer(rec[:8], np.int32)
+    hdr = np.frombuffer(rec[8:8 + 4 * rdt().itemsize], rdt)
+    psi = np.frombuffer(rec[8 + 4 * rdt().itemsize:], cdt).reshape(shape)
+    return coords, hdr, psi
+
+
+C1_16 = dict(Nx=16, Ny=16, Nz=16, Lx=16.0, Ly=16.0, Lz=16.0, init_cond="tf_vortex",
+             mu=10.0, g_phys=500.0, n_lines=1, noise_amp=1e-3, seed=20261203,
+             psi_scale=math.sqrt(1000.0))
+
+
+def test_driver_initial_state_matches_python(tmp_path):
+    """The Fortran seeded initial state equals trapbec_amd.initial's (counter RNG, TF,
+    tanh vortex, noise) to rounding."""
+    _need_driver()
+    nml = tmp_path / "a.nml"
+    _nml(nml, **C1_16, ic_only=True, out_dir=str(tmp_path / "data"))
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    coords, hdr, psi = _read_var(tmp_path / "data" / "proc000" / "var0000.dat", (16, 16, 16))
+    assert list(coords) == [0, 0] and list(hdr[:3]) == [16.0, 16.0, 16.0]
+    _, ref = initial.config_c1(n=16)
+    assert np.abs(psi - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_driver_errors_like_emergency_stop(tmp_path):
+    _need_driver()
+    nml = tmp_path / "bad.nml"
+    _nml(nml, **dict(C1_16, init_cond="no_such_state"), ic_only=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0
+    assert "FATAL ERROR" in r.stdout
+    assert "wrong initial conditions set" in (tmp_path / "ERROR").read_text()
+
+
+@pytest.mark.gpu
+def test_driver_run_matches_oracle(tmp_path):
+    """20 real-time steps through the driver equal the oracle (FP64, 1e-12)."""
+    _need_driver()
+    nml = tmp_path / "run.nml"
+    _nml(nml, **C1_16, nsteps=20, shots=20, out_dir=str(tmp_path / "data"))
+    print(f"\n[case] driver run 16^3", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, hdr, psi = _read_var(tmp_path / "data" / "proc000" / "var0001.dat", (16, 16, 16))
+    pr, psi0 = initial.config_c1(n=16)
+    s = oracle.Stepper(oracle.params(pr.shape, pr.box, pr.kernel_params()["harm_w"],
+                                     dt=1e-3), psi0)
+    s.rk3(20)
+    assert np.abs(psi - s.interior).max() < 1e-12 * np.abs(s.interior).max()
+    assert hdr[3] == pytest.approx(0.02, rel=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_restart_is_bitwise(tmp_path):
+    """Stop after 10 steps, restart from var.dat, run to 20: identical to 20 straight."""
+    _need_driver()
+    print(f"\n[case] driver restart 16^3", flush=True)
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    _nml(a / "run.nml", **C1_16, nsteps=20, shots=10, out_dir=str(a / "data"))
+    assert subprocess.run([DRV, "run.nml"], cwd=a, capture_output=True).returncode == 0
+    _nml(b / "run.nml", **C1_16, nsteps=10, shots=10, out_dir=str(b / "data"))
+    assert subprocess.run([DRV, "run.nml"], cwd=b, capture_output=True).returncode == 0
+    _nml(b / "run2.nml", **dict(C1_16, init_cond="restart"), nsteps=20, shots=10,
+         out_dir=str(b / "data"))
+    r = subprocess.run([DRV, "run2.nml"], cwd=b, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, ha, pa = _read_var(a / "data" / "proc000" / "var0002.dat", (16, 16, 16))
+    _, hb, pb = _read_var(b / "data" / "proc000" / "var0002.dat", (16, 16, 16))
+    assert np.array_equal(pa, pb)
+    assert ha[3] == hb[3]

@@ -1,6 +1,10 @@
 """Build libtbec.so (hipcc, gfx950) in-tree.
 
-    python -m trapbec_amd.build
+The stage kernel's instantiations are split over translation units, one per
+(precision, tile shape) pair (csrc/stage_inst.hip compiled with -DINST_PREC/-DINST_TILE),
+and compiled in parallel, then linked with the host-side sources.
+
+    python -m trapbec_amd.build [-v]
 """
 from __future__ import annotations
 
@@ -8,12 +12,15 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtbec.so")
 SOURCES = ["tbec.hip", "tbec_step.hip", "tbec_batch.hip", "tbec_mlp.hip"]
+INSTANCES = [(p, t) for p in (32, 64) for t in range(4)]
 
 # Packed FP32 instructions are disabled for every HIP file: hipcc would otherwise emit
 # them from ordinary float code.
@@ -29,16 +36,37 @@ def hipcc() -> str:
     raise FileNotFoundError("hipcc not found")
 
 
-def build_lib(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPFLAGS, "-shared", *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed:\n{res.stderr[-6000:]}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"{' '.join(cmd)}\nfailed:\n{res.stderr[-6000:]}")
+    return cmd
+
+
+def build_lib(verbose: bool = False, jobs: int = 0) -> str:
+    cc = hipcc()
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    tmpd = tempfile.mkdtemp(prefix="tbec_build_")
+    try:
+        cmds = []
+        for s in SOURCES:
+            src = os.path.join(CSRC, s)
+            if os.path.exists(src):
+                cmds.append([cc, *HIPFLAGS, "-c", src, "-o", os.path.join(tmpd, s + ".o")])
+        inst = os.path.join(CSRC, "stage_inst.hip")
+        for p, t in INSTANCES:
+            cmds.append([cc, *HIPFLAGS, f"-DINST_PREC={p}", f"-DINST_TILE={t}", "-c", inst,
+                         "-o", os.path.join(tmpd, f"stage_p{p}_t{t}.o")])
+        with ThreadPoolExecutor(jobs) as ex:
+            for cmd in ex.map(_run, cmds):
+                if verbose:
+                    print(" ".join(cmd), flush=True)
+        objs = [c[-1] for c in cmds]
+        tmp = LIB + ".tmp"
+        _run([cc, "--offload-arch=gfx950", "-shared", *objs, "-o", tmp])
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(tmpd, ignore_errors=True)
     return LIB
 
 

@@ -29,13 +29,18 @@
 //              TwoSum (s, e); d_k = (s - 2 f0) + e (the subtraction is exact by
 //              Sterbenz for smooth data), D2 = (270 d1 - 27 d2 + 2 d3)/(180 h^2).
 //              The rounding is then relative to h^2 f'' instead of to f.
+//   SUM_CENTRED: d_k = (f[-k] - f0) + (f[k] - f0): both differences are exact (Sterbenz)
+//              for smooth data and the one rounding of the sum is relative to d_k, the
+//              same error as SUM_COMP at less than half the instructions.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace tbec {
 
-enum SumMode { SUM_REF = 0, SUM_COMP = 1 };
+enum SumMode { SUM_REF = 0, SUM_COMP = 1, SUM_CENTRED = 2 };
 enum StageKind { ST_RK1 = 0, ST_RK2 = 1, ST_RK3 = 2, ST_EULER = 3, ST_RHS = 4 };
 
 template <typename T> struct alignas(2 * sizeof(T)) cplx { T re, im; };
@@ -121,13 +126,18 @@ __device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
   if constexpr (SUM == SUM_REF) {
     return (((((T(2) * m3 - T(27) * m2) + T(270) * m1) - T(490) * c) + T(270) * p1) -
             T(27) * p2) + T(2) * p3;
-  } else {
+  } else if constexpr (SUM == SUM_COMP) {
     T s1, e1, s2, e2, s3, e3;
     two_sum(m1, p1, s1, e1);
     two_sum(m2, p2, s2, e2);
     two_sum(m3, p3, s3, e3);
     const T c2 = c + c;
     const T d1 = (s1 - c2) + e1, d2 = (s2 - c2) + e2, d3 = (s3 - c2) + e3;
+    return T(270) * d1 - T(27) * d2 + T(2) * d3;
+  } else {
+    const T d1 = (m1 - c) + (p1 - c);
+    const T d2 = (m2 - c) + (p2 - c);
+    const T d3 = (m3 - c) + (p3 - c);
     return T(270) * d1 - T(27) * d2 + T(2) * d3;
   }
 }
@@ -239,10 +249,11 @@ gpe_stage(const StageArgs a) {
     }
   }
 
-  // register queue: q[0..6] = planes x-3..x+3 of the own VZ points
-  C q[7][VZ];
-  C nxt[VZ];
-  C hcur[VZ], hnxt[VZ];
+  // Register queue of 8 plane slots: plane p lives in slot (p - xs + 3) & 7.  The loop
+  // is unrolled by 8 so every slot index is a compile-time constant (no register moves
+  // when the queue advances) and the LDS buffer / halo register parity is static too.
+  C q[8][VZ];
+  C hb[2][VZ];
   auto ldv = [&](const C *p, C *o) {
     VecIO<T, VZ>::ld(p, o);
     if constexpr (IMAG) {
@@ -252,7 +263,7 @@ gpe_stage(const StageArgs a) {
   };
 #pragma unroll
   for (int k = 0; k < 7; ++k) ldv(pin + (long long)(xs - 3 + k) * sx + own, q[k]);
-  if (hl) ldv(pin + (long long)xs * sx + hoff, hcur);
+  if (hl) ldv(pin + (long long)xs * sx + hoff, hb[0]);
 
   const T k2x = T(a.k2[0]), k2y = T(a.k2[1]), k2z = T(a.k2[2]);
   const T k1x = T(a.k1[0]), k1y = T(a.k1[1]);
@@ -261,21 +272,30 @@ gpe_stage(const StageArgs a) {
   const T cre = T(a.cre), cim = T(a.cim), A = T(a.A), B = T(a.B);
   const int stage = a.stage;
   const T yc = T(a.y0 + a.hy * y);
-  T zc[VZ];
+  T zc2[VZ];
 #pragma unroll
-  for (int v = 0; v < VZ; ++v) zc[v] = T(a.z0 + a.hz * (z + v));
+  for (int v = 0; v < VZ; ++v) {
+    const T zc = T(a.z0 + a.hz * (z + v));
+    zc2[v] = wlz * (zc * zc);
+  }
+  const T poty = wly * (yc * yc);
   bool zin[VZ];
+  bool full = true;
 #pragma unroll
-  for (int v = 0; v < VZ; ++v) zin[v] = (z + v) < a.nz;
+  for (int v = 0; v < VZ; ++v) {
+    zin[v] = (z + v) < a.nz;
+    full = full && zin[v];
+  }
   const bool yin = y < a.ny;
 
   double red[4] = {0.0, 0.0, 0.0, 0.0};
 
-  for (int x = xs; x < xe; ++x) {
-    const int buf = (x - xs) & 1;
-    // prefetch plane x+4 (own) and x+1 (halo)
-    ldv(pin + (long long)(x + 4) * sx + own, nxt);
-    if (hl) ldv(pin + (long long)(x + 1) * sx + hoff, hnxt);
+  auto plane = [&](auto Rc, const int x) {
+    constexpr int R = decltype(Rc)::value;   // (x - xs) & 7
+    constexpr int buf = R & 1;
+    // prefetch plane x+4 (own) and the halo of plane x+1
+    ldv(pin + (long long)(x + 4) * sx + own, q[(R + 7) & 7]);
+    if (hl) ldv(pin + (long long)(x + 1) * sx + hoff, hb[buf ^ 1]);
     C wv[VZ];
     if (stage == ST_RK2 || stage == ST_RK3) VecIO<T, VZ>::ld(pw + (long long)x * sx + own, wv);
 
@@ -283,17 +303,17 @@ gpe_stage(const StageArgs a) {
     {
       C *row = &lds[buf][3 + ty][4 + tz * VZ];
 #pragma unroll
-      for (int v = 0; v < VZ; ++v) row[v] = q[3][v];
+      for (int v = 0; v < VZ; ++v) row[v] = q[(R + 3) & 7][v];
       if (hl) {
         C *hr = &lds[buf][hrow][hcol];
 #pragma unroll
-        for (int v = 0; v < VZ; ++v) hr[v] = hcur[v];
+        for (int v = 0; v < VZ; ++v) hr[v] = hb[buf][v];
       }
     }
     __syncthreads();
 
     const T xc = T(a.x0 + a.hx * x);
-    const T potxy = wlx * (xc * xc) + wly * (yc * yc);
+    const T potxy = wlx * (xc * xc) + poty;
     // z row segment: columns tz*VZ .. tz*VZ+VZ+7  (z-4 .. z+VZ+3)
     C zr[VZ + 8];
     {
@@ -308,15 +328,15 @@ gpe_stage(const StageArgs a) {
       const C ym3 = lds[buf][ty][col], ym2 = lds[buf][ty + 1][col],
               ym1 = lds[buf][ty + 2][col], yp1 = lds[buf][ty + 4][col],
               yp2 = lds[buf][ty + 5][col], yp3 = lds[buf][ty + 6][col];
-      const C f = q[3][v];
+      const C xm3 = q[R & 7][v], xm2 = q[(R + 1) & 7][v], xm1 = q[(R + 2) & 7][v];
+      const C f = q[(R + 3) & 7][v];
+      const C xp1 = q[(R + 4) & 7][v], xp2 = q[(R + 5) & 7][v], xp3 = q[(R + 6) & 7][v];
       const C zm3 = zr[v + 1], zm2 = zr[v + 2], zm1 = zr[v + 3], zp1 = zr[v + 5],
               zp2 = zr[v + 6], zp3 = zr[v + 7];
       T rr, ri;
       {
-        const T dxr = d2sum<T, SUM>(q[0][v].re, q[1][v].re, q[2][v].re, f.re, q[4][v].re,
-                                    q[5][v].re, q[6][v].re);
-        const T dxi = d2sum<T, SUM>(q[0][v].im, q[1][v].im, q[2][v].im, f.im, q[4][v].im,
-                                    q[5][v].im, q[6][v].im);
+        const T dxr = d2sum<T, SUM>(xm3.re, xm2.re, xm1.re, f.re, xp1.re, xp2.re, xp3.re);
+        const T dxi = d2sum<T, SUM>(xm3.im, xm2.im, xm1.im, f.im, xp1.im, xp2.im, xp3.im);
         const T dyr = d2sum<T, SUM>(ym3.re, ym2.re, ym1.re, f.re, yp1.re, yp2.re, yp3.re);
         const T dyi = d2sum<T, SUM>(ym3.im, ym2.im, ym1.im, f.im, yp1.im, yp2.im, yp3.im);
         const T dzr = d2sum<T, SUM>(zm3.re, zm2.re, zm1.re, f.re, zp1.re, zp2.re, zp3.re);
@@ -327,17 +347,15 @@ gpe_stage(const StageArgs a) {
       if constexpr (ROT) {
         const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
         const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
-        const T d1xr = d1sum<T>(q[0][v].re, q[1][v].re, q[2][v].re, q[4][v].re, q[5][v].re,
-                                q[6][v].re) * k1x;
-        const T d1xi = d1sum<T>(q[0][v].im, q[1][v].im, q[2][v].im, q[4][v].im, q[5][v].im,
-                                q[6][v].im) * k1x;
+        const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
+        const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
         const T pr = xc * d1yr - yc * d1xr;
         const T pim = xc * d1yi - yc * d1xi;
         rr -= om * pim;   // (0, om) * (pr, pim)
         ri += om * pr;
       }
       const T n2 = f.re * f.re + f.im * f.im;
-      const T pot = potxy + wlz * (zc[v] * zc[v]);
+      const T pot = potxy + zc2[v];
       const T fac = (c0 - gnl * n2) - pot;
       rr += fac * f.re;
       ri += fac * f.im;
@@ -369,9 +387,6 @@ gpe_stage(const StageArgs a) {
     // stores (interior points only; padding stays zero)
     if (yin) {
       const long long o = (long long)x * sx + own;
-      bool full = true;
-#pragma unroll
-      for (int v = 0; v < VZ; ++v) full = full && zin[v];
       if (full) {
         if (stage == ST_RHS) {
           VecIO<T, VZ>::st(pw + o, wo);
@@ -392,17 +407,25 @@ gpe_stage(const StageArgs a) {
         }
       }
     }
-    // rotate the queue
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-#pragma unroll
-      for (int v = 0; v < VZ; ++v) q[k][v] = q[k + 1][v];
-    }
-#pragma unroll
-    for (int v = 0; v < VZ; ++v) {
-      q[6][v] = nxt[v];
-      hcur[v] = hnxt[v];
-    }
+  };
+
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  using I5 = std::integral_constant<int, 5>;
+  using I6 = std::integral_constant<int, 6>;
+  using I7 = std::integral_constant<int, 7>;
+  for (int x = xs; x < xe; x += 8) {
+    plane(I0{}, x);
+    if (x + 1 < xe) plane(I1{}, x + 1);
+    if (x + 2 < xe) plane(I2{}, x + 2);
+    if (x + 3 < xe) plane(I3{}, x + 3);
+    if (x + 4 < xe) plane(I4{}, x + 4);
+    if (x + 5 < xe) plane(I5{}, x + 5);
+    if (x + 6 < xe) plane(I6{}, x + 6);
+    if (x + 7 < xe) plane(I7{}, x + 7);
   }
   if constexpr (IMAG) {
     if (a.partials) block_sum4<NT>(red, a.partials + 4LL * lb);

@@ -1,0 +1,62 @@
+// Launchers of the stage kernel, one per (precision, tile shape).  Each is compiled in
+// its own translation unit (stage_inst.hip, built once per pair) so the kernel
+// instantiations compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gpe_stage.hpp"
+
+namespace tbec {
+
+// Tile shapes (TZ, TY, VZ) per precision; the id is the index here.
+template <int PREC, int ID> struct TileShape;
+template <> struct TileShape<32, 0> { static constexpr int TZ = 64, TY = 16, VZ = 4; };
+template <> struct TileShape<32, 1> { static constexpr int TZ = 64, TY = 32, VZ = 4; };
+template <> struct TileShape<32, 2> { static constexpr int TZ = 128, TY = 8, VZ = 4; };
+template <> struct TileShape<32, 3> { static constexpr int TZ = 32, TY = 32, VZ = 4; };
+template <> struct TileShape<64, 0> { static constexpr int TZ = 32, TY = 16, VZ = 2; };
+template <> struct TileShape<64, 1> { static constexpr int TZ = 64, TY = 8, VZ = 2; };
+template <> struct TileShape<64, 2> { static constexpr int TZ = 32, TY = 32, VZ = 2; };
+template <> struct TileShape<64, 3> { static constexpr int TZ = 16, TY = 16, VZ = 2; };
+
+#define TBEC_DECL_LAUNCHER(P, I) \
+  hipError_t launch_stage_p##P##_t##I(const StageArgs &a, int sum, bool rot, bool imag, \
+                                      hipStream_t s);
+TBEC_DECL_LAUNCHER(32, 0)
+TBEC_DECL_LAUNCHER(32, 1)
+TBEC_DECL_LAUNCHER(32, 2)
+TBEC_DECL_LAUNCHER(32, 3)
+TBEC_DECL_LAUNCHER(64, 0)
+TBEC_DECL_LAUNCHER(64, 1)
+TBEC_DECL_LAUNCHER(64, 2)
+TBEC_DECL_LAUNCHER(64, 3)
+#undef TBEC_DECL_LAUNCHER
+
+template <typename T, int TZ, int TY, int VZ, int S>
+static void launch_sum(const StageArgs &a, bool rot, bool imag, hipStream_t s) {
+  constexpr int NT = (TZ / VZ) * TY;
+  dim3 grid(a.nblocks), block(NT);
+  if (rot) {
+    if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, true, false>), grid, block, 0, s, a);
+  } else {
+    if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, false, false>), grid, block, 0, s, a);
+  }
+}
+
+// FP64 uses the reference summation only (it is the parity path); FP32 has all three.
+template <typename T, int TZ, int TY, int VZ>
+static hipError_t launch_tile(const StageArgs &a, int sum, bool rot, bool imag,
+                              hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    launch_sum<T, TZ, TY, VZ, SUM_REF>(a, rot, imag, s);
+  } else {
+    if (sum == SUM_REF) launch_sum<T, TZ, TY, VZ, SUM_REF>(a, rot, imag, s);
+    else if (sum == SUM_COMP) launch_sum<T, TZ, TY, VZ, SUM_COMP>(a, rot, imag, s);
+    else launch_sum<T, TZ, TY, VZ, SUM_CENTRED>(a, rot, imag, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tbec

@@ -13,6 +13,7 @@
 #include <tuple>
 
 #include "gpe_stage.hpp"
+#include "stage_launch.hpp"
 #include "tbec_internal.hpp"
 
 using namespace tbec;
@@ -58,39 +59,21 @@ const TileCfg &tile_cfg(int prec, int id) {
   return prec == TBEC_F32 ? kTilesF32[id] : kTilesF64[id];
 }
 
-template <typename T, int TZ, int TY, int VZ>
-static hipError_t launch_tile(const StageArgs &a, int sum, bool rot, bool imag,
-                              hipStream_t s) {
-  constexpr int NT = (TZ / VZ) * TY;
-  dim3 grid(a.nblocks), block(NT);
-#define TBEC_L(S, R, I) \
-  hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, R, I>), grid, block, 0, s, a)
-  if (sum == SUM_REF) {
-    if (rot) { if (imag) TBEC_L(SUM_REF, true, true); else TBEC_L(SUM_REF, true, false); }
-    else { if (imag) TBEC_L(SUM_REF, false, true); else TBEC_L(SUM_REF, false, false); }
-  } else {
-    if (rot) { if (imag) TBEC_L(SUM_COMP, true, true); else TBEC_L(SUM_COMP, true, false); }
-    else { if (imag) TBEC_L(SUM_COMP, false, true); else TBEC_L(SUM_COMP, false, false); }
-  }
-#undef TBEC_L
-  return hipGetLastError();
-}
-
 hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool rot,
                         bool imag, hipStream_t s) {
   if (prec == TBEC_F32) {
     switch (tile) {
-      case 0: return launch_tile<float, 64, 16, 4>(a, sum, rot, imag, s);
-      case 1: return launch_tile<float, 64, 32, 4>(a, sum, rot, imag, s);
-      case 2: return launch_tile<float, 128, 8, 4>(a, sum, rot, imag, s);
-      default: return launch_tile<float, 32, 32, 4>(a, sum, rot, imag, s);
+      case 0: return launch_stage_p32_t0(a, sum, rot, imag, s);
+      case 1: return launch_stage_p32_t1(a, sum, rot, imag, s);
+      case 2: return launch_stage_p32_t2(a, sum, rot, imag, s);
+      default: return launch_stage_p32_t3(a, sum, rot, imag, s);
     }
   } else {
     switch (tile) {
-      case 0: return launch_tile<double, 32, 16, 2>(a, sum, rot, imag, s);
-      case 1: return launch_tile<double, 64, 8, 2>(a, sum, rot, imag, s);
-      case 2: return launch_tile<double, 32, 32, 2>(a, sum, rot, imag, s);
-      default: return launch_tile<double, 16, 16, 2>(a, sum, rot, imag, s);
+      case 0: return launch_stage_p64_t0(a, sum, rot, imag, s);
+      case 1: return launch_stage_p64_t1(a, sum, rot, imag, s);
+      case 2: return launch_stage_p64_t2(a, sum, rot, imag, s);
+      default: return launch_stage_p64_t3(a, sum, rot, imag, s);
     }
   }
 }
@@ -400,8 +383,10 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
     return fail(TBEC_ERR_ARG, "grid must be positive (got %d x %d x %d)", p->nx, p->ny, p->nz);
   if (p->prec != TBEC_F32 && p->prec != TBEC_F64)
     return fail(TBEC_ERR_ARG, "prec must be 32 or 64 (got %d)", p->prec);
-  if (p->sum_mode != TBEC_SUM_REFERENCE && p->sum_mode != TBEC_SUM_COMPENSATED)
+  if (p->sum_mode < TBEC_SUM_REFERENCE || p->sum_mode > TBEC_SUM_CENTRED)
     return fail(TBEC_ERR_ARG, "unknown sum_mode %d", p->sum_mode);
+  if (p->prec == TBEC_F64 && p->sum_mode != TBEC_SUM_REFERENCE)
+    return fail(TBEC_ERR_ARG, "FP64 runs use TBEC_SUM_REFERENCE");
   if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0))
     return fail(TBEC_ERR_ARG, "box lengths must be positive");
   if (p->tile < -1 || p->tile >= kNumTiles)
