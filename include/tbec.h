@@ -118,6 +118,15 @@ int tbec_diagnose(tbec_handle *h, tbec_diag *out);
 int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
                     int check_every, tbec_solve_result *out);
 
+/* Vortex cores of the current psi by phase winding around grid plaquettes, counting
+ * plaquettes whose corners all have |psi|^2 >= dens_min.  orientation 0: z-normal
+ * (x,y) plaquettes, 1: x-normal (y,z), 2: y-normal (z,x).  Writes up to cap entries
+ * (i, j, k, charge) -- the plaquette's lowest corner -- in (i, j, k) order and the total
+ * number found in *count.  No reference counterpart in the solver; replaces the MATLAB
+ * post-processing detectors (bin/vortex_detect.m, bin/gpeget2dvort_homg.m). */
+int tbec_vortex_cores(tbec_handle *h, double dens_min, int orientation, int32_t *out,
+                      int64_t cap, int64_t *count);
+
 /* Set the norm target used by TBEC_IMAG_RK3 steps (default 1). */
 int tbec_set_norm(tbec_handle *h, double norm);
 
@@ -131,6 +140,47 @@ void *tbec_stream(tbec_handle *h);
 /* Average duration (ms) of one stage kernel measured with HIP events over n stages
  * launched back to back on the handle's stream (real-time RK3 stages). */
 int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage);
+
+/* ---- batched ensembles: many independent condensates advanced together ---------
+ * No reference counterpart (the reference runs one condensate per program).  All
+ * instances share precision, summation mode and tile shape; grids, boxes, physics
+ * parameters and dt may differ.  Each stage of every instance runs in one launch over
+ * (instance, tile) work items; steps are captured into graphs; converged instances are
+ * skipped by a per-instance flag. */
+typedef struct tbec_batch tbec_batch;
+int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out);
+int tbec_batch_destroy(tbec_batch *b);
+int tbec_batch_upload_psi(tbec_batch *b, int32_t i, const void *psi_host);
+int tbec_batch_download_psi(tbec_batch *b, int32_t i, void *psi_host);
+/* Imaginary-time ground states of all instances (TBEC_IMAG_RK3 semantics, each with its
+ * own dtau = -dt_im); out receives n results. */
+int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double norm,
+                          int check_every, tbec_solve_result *out);
+int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg);
+void *tbec_batch_stream(tbec_batch *b);
+
+/* ---- learned warm start ---------------------------------------------------------
+ * No reference counterpart.  An MLP n_in -> depth x width (GELU) -> n_out whose GEMMs run
+ * on BF16 MFMA with FP32 accumulation; FP32 master weights, Adam.  Parameters are laid
+ * out per layer as W (fan_in x fan_out, row-major) then b (fan_out). */
+typedef struct tbec_mlp tbec_mlp;
+int tbec_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, uint64_t seed,
+                    tbec_mlp **out);
+int tbec_mlp_destroy(tbec_mlp *m);
+int64_t tbec_mlp_num_params(tbec_mlp *m);
+int tbec_mlp_get_params(tbec_mlp *m, float *out);
+int tbec_mlp_set_params(tbec_mlp *m, const float *in);
+/* y (batch x n_out) = MLP(x (batch x n_in)); host buffers. */
+int tbec_mlp_forward(tbec_mlp *m, const float *x, int32_t batch, float *y);
+/* Mean-squared-error loss against t and its gradient w.r.t. every parameter. */
+int tbec_mlp_loss_grad(tbec_mlp *m, const float *x, const float *t, int32_t batch,
+                       double *loss, float *grads);
+int tbec_mlp_set_data(tbec_mlp *m, const float *x, const float *t, int32_t n);
+int tbec_mlp_train(tbec_mlp *m, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                   double *loss_hist);
+/* Replace psi by the trilinear upsampling of a coarse (3, cn, cn, cn) field
+ * (sqrt-density, cos phase, sin phase) on the same box; resets the imaginary-time state. */
+int tbec_upload_coarse(tbec_handle *h, const float *coarse, int32_t cn);
 
 const char *tbec_last_error(void);
 /* Library version string and the gfx target it was built for. */

@@ -58,7 +58,7 @@ def lib():
         L.orc_sums_f64.argtypes = [P, vp, vp, vp]
         L.orc_energy_f64.argtypes = [P, vp, vp, C.c_double, vp]
         L.orc_imag_rk3_step_f64.argtypes = [P, vp, vp, vp, C.c_double, vp, vp]
-        L.orc_vortex_plaquettes_f64.argtypes = [P, vp, C.c_double, vp, C.c_long]
+        L.orc_vortex_plaquettes_f64.argtypes = [P, vp, C.c_double, C.c_int, vp, C.c_long]
         L.orc_vortex_plaquettes_f64.restype = C.c_long
         _lib = L
     return _lib
@@ -161,9 +161,9 @@ class Stepper:
                 break
         return hist
 
-    def vortices(self, dens_min: float, cap: int = 1 << 20) -> np.ndarray:
+    def vortices(self, dens_min: float, orientation: int = 0, cap: int = 1 << 20) -> np.ndarray:
         assert self.prec == 64
         out = np.zeros((cap, 4), dtype=np.int32)
         n = lib().orc_vortex_plaquettes_f64(C.byref(self.p), _ptr(self.psi), dens_min,
-                                            _ptr(out), cap)
+                                            orientation, _ptr(out), cap)
         return out[:min(n, cap)].copy()

@@ -174,23 +174,28 @@ void orc_imag_rk3_step_f64(const orc_params *p, double *psi_pad, double *w, doub
   *c0_eff = p->c0 + 2.0 * diag[2];
 }
 
-/* Vortex cores by phase winding around each z-normal grid plaquette (x,y)-(x+1,y)-
- * (x+1,y+1)-(x,y+1) of every z-plane, counting only plaquettes whose four corners
- * have |psi|^2 >= dens_min.  The winding is the sum of the four wrapped phase
- * differences / 2pi (an integer).  Writes up to cap entries (i, j, k, charge) and
- * returns the total count. */
+/* Vortex cores by phase winding around grid plaquettes.  orientation 0: z-normal
+ * plaquettes (x,y)-(x+1,y)-(x+1,y+1)-(x,y+1) in every z-plane; 1: x-normal, corners
+ * (y,z)-(y+1,z)-(y+1,z+1)-(y,z+1) in every x-plane; 2: y-normal, corners
+ * (z,x)-(z+1,x)-(z+1,x+1)-(z,x+1) in every y-plane.  Only plaquettes whose four corners
+ * have |psi|^2 >= dens_min count.  The winding is the sum of the four wrapped phase
+ * differences / 2pi (an integer).  Writes up to cap entries (i, j, k, charge) of the
+ * plaquette's lowest corner, in (i, j, k) order, and returns the total count. */
 long orc_vortex_plaquettes_f64(const orc_params *p, const double *psi_pad, double dens_min,
-                               int *out, long cap) {
+                               int orientation, int *out, long cap) {
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   const long sy = nz + 6, sx = (long)(ny + 6) * (nz + 6);
+  long da, db;   /* offsets of the two in-plane axes, in (first, second) order */
+  int ei = 0, ej = 0, ek = 0;
+  if (orientation == 0) { da = sx; db = sy; ei = 1; ej = 1; }
+  else if (orientation == 1) { da = sy; db = 1; ej = 1; ek = 1; }
+  else { da = 1; db = sx; ek = 1; ei = 1; }
   long cnt = 0;
-  for (int i = 0; i + 1 < nx; ++i)
-    for (int j = 0; j + 1 < ny; ++j)
-      for (int k = 0; k < nz; ++k) {
-        const long o[4] = {(long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3),
-                           (long)(i + 4) * sx + (long)(j + 3) * sy + (k + 3),
-                           (long)(i + 4) * sx + (long)(j + 4) * sy + (k + 3),
-                           (long)(i + 3) * sx + (long)(j + 4) * sy + (k + 3)};
+  for (int i = 0; i + ei < nx; ++i)
+    for (int j = 0; j + ej < ny; ++j)
+      for (int k = 0; k + ek < nz; ++k) {
+        const long o0 = (long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3);
+        const long o[4] = {o0, o0 + da, o0 + da + db, o0 + db};
         double ph[4];
         int ok = 1;
         for (int q = 0; q < 4; ++q) {

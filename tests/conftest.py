@@ -9,6 +9,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import oracle  # noqa: E402  (the CPU checker; tests are its only importers)
+import mlp_oracle  # noqa: E402,F401
 
 oracle.build()
 

@@ -1,0 +1,40 @@
+"""Batched ensembles (many condensates per launch) against one-at-a-time solves."""
+import numpy as np
+import pytest
+
+from trapbec_amd import initial, tbec
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    grids = ((24, 24, 24), (32, 20, 28), (16, 16, 40))
+    return initial.config_c4(seed=7, n_inst=5, dx=0.4, dtau=4e-3, grids=grids)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_equals_single_solves(prec):
+    """Each instance of a batch solve ends bitwise equal to its own single-handle solve
+    (same kernels, same tiles, same fixed-order reductions), with the same step count."""
+    probs = _problems()
+    print(f"\n[case] batch of {len(probs)} prec={prec}", flush=True)
+    with tbec.Batch([p for p, _ in probs], prec=prec) as b:
+        for i, (_, psi0) in enumerate(probs):
+            b.upload(i, psi0)
+        res = b.imag_solve(600, 1e-9, 1.0, 40)
+        got = [b.download(i) for i in range(len(probs))]
+    for i, (pr, psi0) in enumerate(probs):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.upload(psi0)
+            r = c.imag_solve(600, 1e-9, 1.0, 40)
+            ref = c.download()
+        assert r["steps"] == res[i]["steps"], i
+        assert r["converged"] == res[i]["converged"]
+        assert np.array_equal(got[i], ref), (i, np.abs(got[i] - ref).max())
+        assert res[i]["energy"] == r["energy"]
+
+
+def test_batch_rejects_mixed_precision():
+    probs = _problems()[:2]
+    with pytest.raises(tbec.TbecError):
+        tbec.Batch([p for p, _ in probs], prec=48)

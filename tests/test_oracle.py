@@ -292,3 +292,23 @@ def test_vortex_plaquettes_lines():
         j = int(np.searchsorted(x, y0)) - 1
         hits = v[(v[:, 0] == i) & (v[:, 1] == j)]
         assert len(hits) == shape[2] and np.all(hits[:, 3] == q), (x0, y0, h)
+
+
+def test_vortex_plaquettes_ring_orientations():
+    """A vortex ring in the z=0 plane pierces x-normal and y-normal plaquettes (two
+    opposite charges on each axis through the centre) and no z-normal one."""
+    shape, box = (40, 40, 40), (12.0, 12.0, 12.0)
+    f = initial.tf_profile(shape, box, (0.3, 0.3, 0.3), 1.0, mu=10.0)[0] * \
+        initial.vortex_ring_factor(shape, box, 3.0, 0.0, 0.4)
+    s = oracle.Stepper(_lap_only(shape, box), f)
+    assert len(s.vortices(1e-6, 0)) == 0
+    vx = s.vortices(1e-6, 1)       # x-normal plaquettes: the ring cuts the x = const planes
+    vy = s.vortices(1e-6, 2)
+    assert len(vx) > 0 and len(vy) > 0
+    assert set(np.unique(vx[:, 3])) == {-1, 1} and set(np.unique(vy[:, 3])) == {-1, 1}
+    x = initial.coords(40, 12.0)
+    # every x-normal hit sits on the ring: rho(y, z=0) ~ 3 within a cell
+    yy = x[vx[:, 1]] + 0.15
+    xx = x[vx[:, 0]]
+    rho = np.sqrt(xx ** 2 + yy ** 2)
+    assert np.all(np.abs(rho - 3.0) < 0.5)

@@ -174,9 +174,10 @@ __device__ __forceinline__ void block_sum4(double v[4], double *out) {
 }
 
 // ---- the stage kernel ----------------------------------------------------------
+// Process one tile (TZ x TY in z, y; one x segment) of one condensate: logical tile id
+// lb in [0, tiles_z*tiles_y*nseg).  Accumulates this thread's reduction terms in red.
 template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
-__global__ void __launch_bounds__((TZ / VZ) * TY)
-gpe_stage(const StageArgs a) {
+__device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, double red[4]) {
   using C = cplx<T>;
   constexpr int NTZ = TZ / VZ;
   constexpr int NT = NTZ * TY;
@@ -187,16 +188,6 @@ gpe_stage(const StageArgs a) {
   constexpr int LY = TY + 6;
   __shared__ C lds[2][LY][LZ];
 
-  if constexpr (IMAG) {
-    if (a.skip && *a.skip) return;
-  }
-
-  // XCD-aware tile order: hardware id b lands on XCD b % 8; give XCD k the logical
-  // range [k*per, (k+1)*per).
-  const int nb = a.nblocks;
-  const int b = blockIdx.x;
-  int lb = b;
-  if ((nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
   const int tiles = a.tiles_z * a.tiles_y;
   const int seg = lb / tiles;
   const int t2 = lb - seg * tiles;
@@ -287,8 +278,6 @@ gpe_stage(const StageArgs a) {
     full = full && zin[v];
   }
   const bool yin = y < a.ny;
-
-  double red[4] = {0.0, 0.0, 0.0, 0.0};
 
   auto plane = [&](auto Rc, const int x) {
     constexpr int R = decltype(Rc)::value;   // (x - xs) & 7
@@ -427,8 +416,76 @@ gpe_stage(const StageArgs a) {
     if (x + 6 < xe) plane(I6{}, x + 6);
     if (x + 7 < xe) plane(I7{}, x + 7);
   }
+}
+
+template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+__global__ void __launch_bounds__((TZ / VZ) * TY)
+gpe_stage(const StageArgs a) {
+  constexpr int NT = (TZ / VZ) * TY;
+  if constexpr (IMAG) {
+    if (a.skip && *a.skip) return;
+  }
+  // XCD-aware tile order: hardware id b lands on XCD b % 8; give XCD k the logical
+  // range [k*per, (k+1)*per) so neighbouring tiles share that XCD's L2.
+  const int nb = a.nblocks;
+  const int b = blockIdx.x;
+  int lb = b;
+  if ((nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  march_tile<T, TZ, TY, VZ, SUM, ROT, IMAG>(a, lb, red);
   if constexpr (IMAG) {
     if (a.partials) block_sum4<NT>(red, a.partials + 4LL * lb);
+  }
+}
+
+// Many condensates in one launch.  Work items are (instance, tile) pairs; a fixed grid
+// of workgroups walks them with a grid stride.  Per-instance static arguments live in
+// a device table; the launch supplies the stage and the buffer parity.
+struct BatchInst {
+  StageArgs a;          // psi_in/psi_out/w, stage fields are filled per item
+  void *psi[2];
+  double dt_re, dt_im;  // per-instance complex step
+  int item_begin;       // first work item of this instance
+  int pad_;
+};
+
+struct BatchLaunch {
+  const BatchInst *inst;
+  const int *item_inst; // work item -> instance
+  int total_items;
+  int parity;           // psi[parity] is read, psi[parity ^ 1] written
+  int stage;            // StageKind
+  double A, B;
+  double *partials;     // 4 doubles per work item
+  const double *state;  // per-instance state rows (stride state_stride)
+  int state_stride;
+  const int *flags;     // per-instance flags (stride 4); skip when flags[4*i] != 0
+};
+
+template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+__global__ void __launch_bounds__((TZ / VZ) * TY)
+gpe_stage_batch(const BatchLaunch L) {
+  constexpr int NT = (TZ / VZ) * TY;
+  for (int item = blockIdx.x; item < L.total_items; item += gridDim.x) {
+    const int i = L.item_inst[item];
+    if (L.flags[4 * i] != 0) continue;          // converged or failed: skip (uniform)
+    const BatchInst &bi = L.inst[i];
+    StageArgs a = bi.a;
+    a.psi_in = bi.psi[L.parity];
+    a.psi_out = bi.psi[L.parity ^ 1];
+    a.stage = L.stage;
+    a.A = L.A;
+    a.B = L.B;
+    if (L.stage == ST_EULER) { a.cre = 0.5 * bi.dt_re; a.cim = 0.5 * bi.dt_im; }
+    else { a.cre = -0.5 * bi.dt_im; a.cim = 0.5 * bi.dt_re; }
+    const double *st = L.state + (long long)i * L.state_stride;
+    a.scale = (IMAG && L.stage == ST_RK1) ? st + 0 : nullptr;   // ST_SCALE
+    a.c0dev = IMAG ? st + 1 : nullptr;                            // ST_C0EFF
+    a.skip = nullptr;
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    march_tile<T, TZ, TY, VZ, SUM, ROT, IMAG>(a, item - bi.item_begin, red);
+    if constexpr (IMAG) block_sum4<NT>(red, L.partials + 4LL * item);
+    __syncthreads();
   }
 }
 

@@ -11,6 +11,7 @@
 
 #define TBEC_CAT4(a, b, c, d) a##b##c##d
 #define TBEC_NAME(P, I) TBEC_CAT4(launch_stage_p, P, _t, I)
+#define TBEC_BNAME(P, I) TBEC_CAT4(launch_batch_p, P, _t, I)
 
 namespace tbec {
 using InstT = std::conditional_t<INST_PREC == 32, float, double>;
@@ -19,5 +20,11 @@ using InstShape = TileShape<INST_PREC, INST_TILE>;
 hipError_t TBEC_NAME(INST_PREC, INST_TILE)(const StageArgs &a, int sum, bool rot, bool imag,
                                            hipStream_t s) {
   return launch_tile<InstT, InstShape::TZ, InstShape::TY, InstShape::VZ>(a, sum, rot, imag, s);
+}
+
+hipError_t TBEC_BNAME(INST_PREC, INST_TILE)(const BatchLaunch &L, int sum, bool rot, int grid,
+                                            hipStream_t s) {
+  return launch_batch_tile<InstT, InstShape::TZ, InstShape::TY, InstShape::VZ>(L, sum, rot,
+                                                                               grid, s);
 }
 }  // namespace tbec

@@ -32,6 +32,23 @@ TBEC_DECL_LAUNCHER(64, 2)
 TBEC_DECL_LAUNCHER(64, 3)
 #undef TBEC_DECL_LAUNCHER
 
+#define TBEC_DECL_BLAUNCHER(P, I)                                                      \
+  hipError_t launch_batch_p##P##_t##I(const BatchLaunch &L, int sum, bool rot, int grid, \
+                                      hipStream_t s);
+TBEC_DECL_BLAUNCHER(32, 0)
+TBEC_DECL_BLAUNCHER(32, 1)
+TBEC_DECL_BLAUNCHER(32, 2)
+TBEC_DECL_BLAUNCHER(32, 3)
+TBEC_DECL_BLAUNCHER(64, 0)
+TBEC_DECL_BLAUNCHER(64, 1)
+TBEC_DECL_BLAUNCHER(64, 2)
+TBEC_DECL_BLAUNCHER(64, 3)
+#undef TBEC_DECL_BLAUNCHER
+
+// Dispatch to the (precision, tile) launcher.
+hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool rot, int grid,
+                        hipStream_t s);
+
 template <typename T, int TZ, int TY, int VZ, int S>
 static void launch_sum(const StageArgs &a, bool rot, bool imag, hipStream_t s) {
   constexpr int NT = (TZ / VZ) * TY;
@@ -55,6 +72,31 @@ static hipError_t launch_tile(const StageArgs &a, int sum, bool rot, bool imag,
     if (sum == SUM_REF) launch_sum<T, TZ, TY, VZ, SUM_REF>(a, rot, imag, s);
     else if (sum == SUM_COMP) launch_sum<T, TZ, TY, VZ, SUM_COMP>(a, rot, imag, s);
     else launch_sum<T, TZ, TY, VZ, SUM_CENTRED>(a, rot, imag, s);
+  }
+  return hipGetLastError();
+}
+
+// Batched imaginary-time stages (reductions always on).
+template <typename T, int TZ, int TY, int VZ, int S>
+static void launch_batch_sum(const BatchLaunch &L, bool rot, int grid, hipStream_t s) {
+  constexpr int NT = (TZ / VZ) * TY;
+  if (rot)
+    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, true, true>), dim3(grid), dim3(NT), 0,
+                       s, L);
+  else
+    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, false, true>), dim3(grid), dim3(NT),
+                       0, s, L);
+}
+
+template <typename T, int TZ, int TY, int VZ>
+static hipError_t launch_batch_tile(const BatchLaunch &L, int sum, bool rot, int grid,
+                                    hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    launch_batch_sum<T, TZ, TY, VZ, SUM_REF>(L, rot, grid, s);
+  } else {
+    if (sum == SUM_REF) launch_batch_sum<T, TZ, TY, VZ, SUM_REF>(L, rot, grid, s);
+    else if (sum == SUM_COMP) launch_batch_sum<T, TZ, TY, VZ, SUM_COMP>(L, rot, grid, s);
+    else launch_batch_sum<T, TZ, TY, VZ, SUM_CENTRED>(L, rot, grid, s);
   }
   return hipGetLastError();
 }

@@ -15,6 +15,7 @@
 #include "gpe_stage.hpp"
 #include "stage_launch.hpp"
 #include "tbec_internal.hpp"
+#include "tbec_fin.hpp"
 
 using namespace tbec;
 
@@ -161,101 +162,25 @@ k_info(const cplx<T> *__restrict__ p, int nx, int ny, int nz, long long sy, long
   }
 }
 
-// Fixed-order sum of nb x 4 partials with one 256-thread workgroup.
-__device__ void sum_partials(const double *part, int nb, double out[4], bool minmax) {
-  __shared__ double red[4][256];
-  double v[4];
-  if (minmax) { v[0] = 0.0; v[1] = INFINITY; v[2] = 0.0; v[3] = 0.0; }
-  else { v[0] = v[1] = v[2] = v[3] = 0.0; }
-  for (int i = threadIdx.x; i < nb; i += 256) {
-    if (minmax) {
-      v[0] = fmax(v[0], part[4 * i]);
-      v[1] = fmin(v[1], part[4 * i + 1]);
-    } else {
-      v[0] += part[4 * i];
-      v[1] += part[4 * i + 1];
-    }
-    v[2] += part[4 * i + 2];
-    v[3] += part[4 * i + 3];
-  }
-  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      if (minmax) {
-        red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-        red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][threadIdx.x + o]);
-      } else {
-        red[0][threadIdx.x] += red[0][threadIdx.x + o];
-        red[1][threadIdx.x] += red[1][threadIdx.x + o];
-      }
-      red[2][threadIdx.x] += red[2][threadIdx.x + o];
-      red[3][threadIdx.x] += red[3][threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  for (int q = 0; q < 4; ++q) out[q] = red[q][0];
-}
-
-// After a stage-1 (or rhs-only) pass with reductions: energy, mu, residual; the
-// convergence test; c0 for the next step.  st[] layout in tbec_internal.hpp.
 __global__ void __launch_bounds__(256)
 k_fin_energy(const double *part, int nb, double *st, int *flag, int check_conv) {
   if (check_conv && flag[0]) return;
   double s[4];
-  sum_partials(part, nb, s, false);
-  if (threadIdx.x != 0) return;
-  const double dV = st[ST_DV];
-  const double N = s[0] * dV, S1 = s[1] * dV, S4 = s[2] * dV, R = s[3] * dV;
-  const double c0 = check_conv ? st[ST_C0EFF] : st[ST_C0BASE];
-  const double H = -0.5 * S1 + 0.5 * c0 * N;
-  const double E = H - 0.25 * st[ST_GNL] * S4;
-  const double mu = H / N;
-  const double am = 0.5 * c0 - mu;
-  double r2 = 0.25 * R - am * S1 + am * am * N;
-  if (r2 < 0) r2 = 0;
-  const double res = sqrt(r2 / N);
-  st[ST_N] = N;
-  st[ST_E] = E;
-  st[ST_MU] = mu;
-  st[ST_RES] = res;
-  if (!check_conv) return;
-  if (!isfinite(E) || !isfinite(N) || N <= 0.0) {
-    flag[0] = 2;
-    flag[1] = (int)st[ST_STEP];
-    return;
-  }
-  const double rel = st[ST_STEP] > 0.0 ? fabs(E - st[ST_EPREV]) / fabs(E) : INFINITY;
-  st[ST_REL] = rel;
-  st[ST_EPREV] = E;
-  st[ST_C0NEXT] = st[ST_C0BASE] + 2.0 * mu;
-  if (rel < st[ST_TOL]) {
-    flag[0] = 1;
-    flag[1] = (int)st[ST_STEP];
-  }
+  sum_partials(part, 0, nb, s, false);
+  if (threadIdx.x == 0) fin_energy(s, st, flag, check_conv);
 }
 
-// After stage 3 with reductions: renormalisation scale for the next read, c0 update.
 __global__ void __launch_bounds__(256)
 k_fin_norm(const double *part, int nb, double *st, int *flag) {
   if (flag[0]) return;
   double s[4];
-  sum_partials(part, nb, s, false);
-  if (threadIdx.x != 0) return;
-  const double N = s[0] * st[ST_DV];
-  if (!isfinite(N) || N <= 0.0) {
-    flag[0] = 2;
-    flag[1] = (int)st[ST_STEP];
-    return;
-  }
-  st[ST_SCALE] = sqrt(st[ST_NORMT] / N);
-  st[ST_C0EFF] = st[ST_C0NEXT];
-  st[ST_STEP] += 1.0;
+  sum_partials(part, 0, nb, s, false);
+  if (threadIdx.x == 0) fin_norm(s, st, flag);
 }
 
 __global__ void __launch_bounds__(256) k_fin_info(const double *part, int nb, double *st) {
   double s[4];
-  sum_partials(part, nb, s, true);
+  sum_partials(part, 0, nb, s, true);
   if (threadIdx.x != 0) return;
   st[ST_MAXA] = s[0];
   st[ST_MINA] = s[1];
@@ -267,46 +192,87 @@ __global__ void __launch_bounds__(256) k_fin_info(const double *part, int nb, do
 
 // ------------------------------------------------------------------ handle ------
 namespace tbec {
-static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
-static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A, double B,
-                     double cre, double cim) {
-  const TileCfg &tc = tile_cfg(h->p.prec, h->tile);
-  a.psi_in = h->psi[src];
-  a.psi_out = h->psi[src ^ 1];
-  a.w = h->w;
-  a.nx = h->p.nx; a.ny = h->p.ny; a.nz = h->p.nz;
-  a.seg_len = h->seg;
-  a.tiles_z = (int)((h->p.nz + tc.tz - 1) / tc.tz);
-  a.tiles_y = (int)((h->p.ny + tc.ty - 1) / tc.ty);
-  a.nseg = (h->p.nx + h->seg - 1) / h->seg;
+// Padded device layout of one condensate: interior z starts at a 128-byte boundary;
+// tiles of up to 128 in z and 64 in y may overrun the interior and read zeros; x has 3
+// ghost planes below and 4 above (the register queue prefetches plane x+4).
+void compute_layout(const tbec_params &p, Layout &L) {
+  L.csize = p.prec == TBEC_F32 ? 8 : 16;
+  L.hx = p.Lx / p.nx;
+  L.hy = p.Ly / p.ny;
+  L.hz = p.Lz / p.nz;
+  const long long zoff = 128 / L.csize;
+  L.PZ = round_up(zoff + round_up(p.nz, 128) + 4, zoff);
+  L.PY = 3 + round_up(p.ny, 64) + 3;
+  L.PX = 3 + (long long)p.nx + 4;
+  L.sy = L.PZ;
+  L.sx = L.PZ * L.PY;
+  L.base = 3 * L.sx + 3 * L.sy + zoff;
+  L.elems = L.sx * L.PX;
+}
+
+// Static stage arguments of one condensate (geometry, physics, tiling); returns the
+// number of tiles.  Buffers, stage fields and reduction pointers are set by the caller.
+int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, StageArgs &a) {
+  const TileCfg &tc = tile_cfg(p.prec, tile);
+  a.psi_in = nullptr;
+  a.psi_out = nullptr;
+  a.w = nullptr;
+  a.nx = p.nx; a.ny = p.ny; a.nz = p.nz;
+  a.seg_len = seg;
+  a.tiles_z = (int)((p.nz + tc.tz - 1) / tc.tz);
+  a.tiles_y = (int)((p.ny + tc.ty - 1) / tc.ty);
+  a.nseg = (p.nx + seg - 1) / seg;
   a.nblocks = a.tiles_z * a.tiles_y * a.nseg;
-  a.sy = h->sy; a.sx = h->sx; a.base = h->base;
-  a.x0 = -0.5 * h->p.Lx + 0.5 * h->hx;
-  a.y0 = -0.5 * h->p.Ly + 0.5 * h->hy;
-  a.z0 = -0.5 * h->p.Lz + 0.5 * h->hz;
-  a.hx = h->hx; a.hy = h->hy; a.hz = h->hz;
-  a.k2[0] = 1.0 / (180.0 * h->hx * h->hx);
-  a.k2[1] = 1.0 / (180.0 * h->hy * h->hy);
-  a.k2[2] = 1.0 / (180.0 * h->hz * h->hz);
-  a.k1[0] = 1.0 / (60.0 * h->hx);
-  a.k1[1] = 1.0 / (60.0 * h->hy);
-  a.wl[0] = h->p.harm_wx / (h->p.Lx * h->p.Lx);
-  a.wl[1] = h->p.harm_wy / (h->p.Ly * h->p.Ly);
-  a.wl[2] = h->p.harm_wz / (h->p.Lz * h->p.Lz);
-  a.c0 = h->p.c0; a.gnl = h->p.gnl; a.om = h->p.omega_z;
-  a.cre = cre; a.cim = cim; a.A = A; a.B = B;
-  a.stage = stage;
+  a.sy = L.sy; a.sx = L.sx; a.base = L.base;
+  a.x0 = -0.5 * p.Lx + 0.5 * L.hx;
+  a.y0 = -0.5 * p.Ly + 0.5 * L.hy;
+  a.z0 = -0.5 * p.Lz + 0.5 * L.hz;
+  a.hx = L.hx; a.hy = L.hy; a.hz = L.hz;
+  a.k2[0] = 1.0 / (180.0 * L.hx * L.hx);
+  a.k2[1] = 1.0 / (180.0 * L.hy * L.hy);
+  a.k2[2] = 1.0 / (180.0 * L.hz * L.hz);
+  a.k1[0] = 1.0 / (60.0 * L.hx);
+  a.k1[1] = 1.0 / (60.0 * L.hy);
+  a.wl[0] = p.harm_wx / (p.Lx * p.Lx);
+  a.wl[1] = p.harm_wy / (p.Ly * p.Ly);
+  a.wl[2] = p.harm_wz / (p.Lz * p.Lz);
+  a.c0 = p.c0; a.gnl = p.gnl; a.om = p.omega_z;
+  a.cre = a.cim = a.A = a.B = 0.0;
+  a.stage = 0;
   a.scale = nullptr; a.c0dev = nullptr; a.skip = nullptr; a.partials = nullptr;
   return a.nblocks;
 }
 
-bool handle_rot(const tbec_handle *h) {
+static Layout layout_of(const tbec_handle *h) {
+  Layout L;
+  L.csize = h->csize;
+  L.hx = h->hx; L.hy = h->hy; L.hz = h->hz;
+  L.PZ = h->PZ; L.PY = h->PY; L.PX = h->PX;
+  L.sy = h->sy; L.sx = h->sx; L.base = h->base; L.elems = h->elems;
+  return L;
+}
+
+static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A, double B,
+                     double cre, double cim) {
+  fill_static_args(h->p, layout_of(h), h->tile, h->seg, a);
+  a.psi_in = h->psi[src];
+  a.psi_out = h->psi[src ^ 1];
+  a.w = h->w;
+  a.cre = cre; a.cim = cim; a.A = A; a.B = B;
+  a.stage = stage;
+  return a.nblocks;
+}
+
+bool rot_enabled(const tbec_params &p) {
   // the reference enables the frame term when |omega_z| > epsilon(0.) of the working
   // precision (src/finite_diff.f90:128)
-  const double eps = h->p.prec == TBEC_F32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
-  return std::fabs(h->p.omega_z) > eps;
+  const double eps = p.prec == TBEC_F32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+  return std::fabs(p.omega_z) > eps;
 }
+
+bool handle_rot(const tbec_handle *h) { return rot_enabled(h->p); }
 
 int tbec_reserve_partials(tbec_handle *h, int nb) {
   if (nb > h->npart) {
@@ -397,24 +363,17 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   h->p = *p;
   h->tile = p->tile < 0 ? 0 : p->tile;
   h->seg = 64;
-  h->csize = p->prec == TBEC_F32 ? 8 : 16;
-  h->hx = p->Lx / p->nx;
-  h->hy = p->Ly / p->ny;
-  h->hz = p->Lz / p->nz;
+  Layout L;
+  compute_layout(*p, L);
+  h->csize = L.csize;
+  h->hx = L.hx;
+  h->hy = L.hy;
+  h->hz = L.hz;
   // dt*(1 - i*gamma) for real time (src/run.f90:179)
   h->dt_re = p->dt_re + p->gamma * p->dt_im;
   h->dt_im = p->dt_im - p->gamma * p->dt_re;
-  // padded layout: interior z starts at a 128-byte boundary; tiles of up to 128 in z and
-  // 64 in y may overrun the interior and read zeros; x has 3 ghost planes below and 4
-  // above (the register queue prefetches plane x+4).
-  const long long zoff = 128 / h->csize;
-  h->PZ = round_up(zoff + round_up(p->nz, 128) + 4, zoff);
-  h->PY = 3 + round_up(p->ny, 64) + 3;
-  h->PX = 3 + (long long)p->nx + 4;
-  h->sy = h->PZ;
-  h->sx = h->PZ * h->PY;
-  h->base = 3 * h->sx + 3 * h->sy + zoff;
-  h->elems = h->sx * h->PX;
+  h->PZ = L.PZ; h->PY = L.PY; h->PX = L.PX;
+  h->sy = L.sy; h->sx = L.sx; h->base = L.base; h->elems = L.elems;
   const size_t bytes = (size_t)h->elems * h->csize;
   hipError_t e1 = hipMalloc(&h->psi[0], bytes);
   hipError_t e2 = e1 == hipSuccess ? hipMalloc(&h->psi[1], bytes) : e1;

@@ -28,6 +28,18 @@ enum StateSlot {
 };
 
 struct TileCfg { int tz, ty, vz; };
+struct StageArgs;
+
+// Padded device layout of one condensate (complex elements).
+struct Layout {
+  long long PZ = 0, PY = 0, PX = 0, sy = 0, sx = 0, base = 0, elems = 0;
+  double hx = 0, hy = 0, hz = 0;
+  int csize = 8;
+};
+void compute_layout(const tbec_params &p, Layout &L);
+int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, StageArgs &a);
+bool rot_enabled(const tbec_params &p);
+long long round_up(long long v, long long m);
 const TileCfg &tile_cfg(int prec, int id);
 extern const int kNumTiles;
 

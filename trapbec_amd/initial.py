@@ -230,3 +230,30 @@ def reference_soliton(shape, box, harm_w):
     ld = np.hypot(tf, tf)
     pi12 = 3.14159265359
     return np.where(X < 0.0, ld + 0j, ld * complex(math.cos(pi12), math.sin(pi12)))
+
+
+C4_GRIDS = ((64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 160), (96, 96, 192))
+
+
+def config_c4(seed: int = 20261203, n_inst: int = 48, dx: float = 0.2, dtau: float = 1e-3,
+              grids=C4_GRIDS):
+    """Ensemble of independent imaginary-time problems: grids drawn by seed from `grids`
+    with spacing dx (box = shape*dx); w_perp in [0.8, 1.2], w_z in [0.3, 1.5],
+    g in [200, 4000], Omega in [0, 0.8], seeded uniform; start = TF (norm 1) times seeded
+    phase noise of amplitude 0.1.  Returns [(Problem, psi0)]."""
+    u = uniform(seed, 301, np.arange(6 * n_inst)).reshape(n_inst, 6)
+    out = []
+    for i in range(n_inst):
+        shape = tuple(grids[min(int(u[i, 0] * len(grids)), len(grids) - 1)])
+        box = tuple(n * dx for n in shape)
+        wp = 0.8 + 0.4 * u[i, 1]
+        wz = 0.3 + 1.2 * u[i, 2]
+        g = 200.0 + 3800.0 * u[i, 3]
+        Om = 0.8 * u[i, 4]
+        pr = Problem(shape, box, (wp, wp, wz), g, Om, -dtau * 1j, False,
+                     {"name": f"c4_{i}", "seed": seed})
+        psi, mu = tf_profile(shape, box, pr.omega, g, norm=1.0)
+        psi = psi * phase_noise(shape, seed + i, 0.1)
+        pr.meta.update(mu=mu)
+        out.append((pr, psi))
+    return out

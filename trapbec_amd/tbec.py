@@ -54,7 +54,13 @@ class TbecError(RuntimeError):
 EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi",
            "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
            "tbec_diagnose", "tbec_imag_solve", "tbec_set_norm", "tbec_sync", "tbec_stream",
-           "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version"]
+           "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version",
+           "tbec_batch_create", "tbec_batch_destroy", "tbec_batch_upload_psi",
+           "tbec_batch_download_psi", "tbec_batch_imag_solve", "tbec_batch_set_tuning",
+           "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
+           "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
+           "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
+           "tbec_upload_coarse"]
 
 _lib = None
 
@@ -86,8 +92,19 @@ def load(path: str = LIB_PATH):
     L.tbec_time_stages.argtypes = [H, C.c_int, C.POINTER(C.c_double)]
     L.tbec_last_error.restype = C.c_char_p
     L.tbec_version.restype = C.c_char_p
+    L.tbec_vortex_cores.argtypes = [H, C.c_double, C.c_int, vp, C.c_int64,
+                                     C.POINTER(C.c_int64)]
+    L.tbec_batch_destroy.argtypes = [H]
+    L.tbec_batch_upload_psi.argtypes = [H, C.c_int32, vp]
+    L.tbec_batch_download_psi.argtypes = [H, C.c_int32, vp]
+    L.tbec_batch_imag_solve.argtypes = [H, C.c_int64, C.c_double, C.c_double, C.c_int,
+                                        C.POINTER(TbecSolveResult)]
+    L.tbec_batch_set_tuning.argtypes = [H, C.c_int, C.c_int]
+    L.tbec_batch_stream.argtypes = [H]
+    L.tbec_batch_stream.restype = C.c_void_p
     for f in EXPORTS:
-        if f not in ("tbec_stream", "tbec_last_error", "tbec_version"):
+        if f not in ("tbec_stream", "tbec_last_error", "tbec_version", "tbec_batch_stream",
+                     "tbec_mlp_num_params"):
             getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -183,6 +200,14 @@ class Condensate:
                                       C.byref(r)))
         return {k: getattr(r, k) for k, _ in TbecSolveResult._fields_ if k != "pad_"}
 
+    def vortex_cores(self, dens_min: float, orientation: int = 0, cap: int = 1 << 20):
+        """(i, j, k, charge) of every plaquette with nonzero phase winding."""
+        out = np.zeros((cap, 4), dtype=np.int32)
+        cnt = C.c_int64()
+        _check(load().tbec_vortex_cores(self._h, dens_min, orientation,
+                                        out.ctypes.data_as(C.c_void_p), cap, C.byref(cnt)))
+        return out[:min(cnt.value, cap)].copy()
+
     def set_norm(self, norm: float):
         _check(load().tbec_set_norm(self._h, norm))
 
@@ -199,3 +224,160 @@ class Condensate:
         ms = C.c_double()
         _check(load().tbec_time_stages(self._h, nstages, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Many independent condensates solved together in imaginary time (libtbec batch).
+
+    `problems` is a list of trapbec_amd.initial.Problem with imaginary dt = (0, -dtau);
+    all share one precision."""
+
+    def __init__(self, problems, prec=32, sum_mode=None, tile=-1):
+        L = load()
+        if sum_mode is None:
+            sum_mode = SUM_REFERENCE if prec == 64 else SUM_COMPENSATED
+        self.n = len(problems)
+        self.prec = prec
+        self.cdt = np.complex128 if prec == 64 else np.complex64
+        self.shapes = [tuple(pr.shape) for pr in problems]
+        arr = (TbecParams * self.n)()
+        for i, pr in enumerate(problems):
+            kp = pr.kernel_params()
+            dt = complex(kp["dt"])
+            arr[i] = TbecParams(pr.shape[0], pr.shape[1], pr.shape[2], prec, sum_mode, tile,
+                                pr.box[0], pr.box[1], pr.box[2], *kp["harm_w"], kp["omega_r"],
+                                kp["c0"], kp["gnl"], dt.real, dt.imag, 0.0)
+        self._params = arr
+        self._h = C.c_void_p()
+        L.tbec_batch_create.argtypes = [C.c_int32, C.POINTER(TbecParams), C.POINTER(C.c_void_p)]
+        _check(L.tbec_batch_create(self.n, arr, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().tbec_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, psi):
+        a = np.ascontiguousarray(psi, dtype=self.cdt)
+        assert a.shape == self.shapes[i]
+        _check(load().tbec_batch_upload_psi(self._h, i, a.ctypes.data_as(C.c_void_p)))
+
+    def download(self, i):
+        out = np.empty(self.shapes[i], dtype=self.cdt)
+        _check(load().tbec_batch_download_psi(self._h, i, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_tuning(self, tile, seg):
+        _check(load().tbec_batch_set_tuning(self._h, tile, seg))
+
+    def imag_solve(self, max_steps, tol, norm=1.0, check_every=50):
+        res = (TbecSolveResult * self.n)()
+        _check(load().tbec_batch_imag_solve(self._h, max_steps, tol, norm, check_every, res))
+        return [{k: getattr(r, k) for k, _ in TbecSolveResult._fields_ if k != "pad_"}
+                for r in res]
+
+    def stream_ptr(self):
+        return load().tbec_batch_stream(self._h)
+
+
+class MLP:
+    """The warm-start network on the GPU (libtbec BF16-MFMA kernels)."""
+
+    def __init__(self, n_in, width, depth, n_out, seed=1):
+        L = load()
+        for f, a in (("tbec_mlp_create", [C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_uint64, C.POINTER(C.c_void_p)]),
+                     ("tbec_mlp_destroy", [C.c_void_p]),
+                     ("tbec_mlp_get_params", [C.c_void_p, C.c_void_p]),
+                     ("tbec_mlp_set_params", [C.c_void_p, C.c_void_p]),
+                     ("tbec_mlp_forward", [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+                     ("tbec_mlp_loss_grad", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                             C.POINTER(C.c_double), C.c_void_p]),
+                     ("tbec_mlp_set_data", [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+                     ("tbec_mlp_train", [C.c_void_p, C.c_int32, C.c_int32, C.c_float,
+                                         C.c_uint64, C.c_void_p]),
+                     ("tbec_mlp_num_params", [C.c_void_p]),
+                     ("tbec_upload_coarse", [C.c_void_p, C.c_void_p, C.c_int32])):
+            getattr(L, f).argtypes = a
+        L.tbec_mlp_num_params.restype = C.c_int64
+        self.dims = [n_in] + [width] * depth + [n_out]
+        self._h = C.c_void_p()
+        _check(L.tbec_mlp_create(n_in, width, depth, n_out, seed, C.byref(self._h)))
+        self.nparam = L.tbec_mlp_num_params(self._h)
+
+    def close(self):
+        if self._h:
+            load().tbec_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_params(self):
+        out = np.empty(self.nparam, np.float32)
+        _check(load().tbec_mlp_get_params(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam
+        _check(load().tbec_mlp_set_params(self._h, p.ctypes.data_as(C.c_void_p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((x.shape[0], self.dims[-1]), np.float32)
+        _check(load().tbec_mlp_forward(self._h, x.ctypes.data_as(C.c_void_p), x.shape[0],
+                                       y.ctypes.data_as(C.c_void_p)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_double()
+        _check(load().tbec_mlp_loss_grad(self._h, x.ctypes.data_as(C.c_void_p),
+                                         t.ctypes.data_as(C.c_void_p), x.shape[0],
+                                         C.byref(loss), g.ctypes.data_as(C.c_void_p)))
+        return loss.value, g
+
+    def set_data(self, x, t):
+        self._x = np.ascontiguousarray(x, np.float32)
+        self._t = np.ascontiguousarray(t, np.float32)
+        _check(load().tbec_mlp_set_data(self._h, self._x.ctypes.data_as(C.c_void_p),
+                                        self._t.ctypes.data_as(C.c_void_p), self._x.shape[0]))
+
+    def train(self, epochs, batch, lr, seed=1):
+        hist = np.zeros(epochs)
+        _check(load().tbec_mlp_train(self._h, epochs, batch, lr, seed,
+                                     hist.ctypes.data_as(C.c_void_p)))
+        return hist
+
+
+def upload_coarse(cond: Condensate, coarse: np.ndarray):
+    """Start `cond` from the trilinear upsampling of a (3, n, n, n) coarse field."""
+    c = np.ascontiguousarray(coarse, np.float32)
+    assert c.ndim == 4 and c.shape[0] == 3
+    L = load()
+    L.tbec_upload_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    _check(L.tbec_upload_coarse(cond._h, c.ctypes.data_as(C.c_void_p), c.shape[1]))
