@@ -45,6 +45,9 @@ def parse():
                     default="compensated")
     ap.add_argument("--variants", action="store_true",
                     help="also time the other FP32 summation modes")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: 3 FP32 stages at n^3 plus a plain "
+                         "device copy of one psi-sized buffer (the counter calibration)")
     return ap.parse_args()
 
 
@@ -75,7 +78,7 @@ def run_gpu(args, prec, steps, psi_dev, pr, sum_name=None):
     from trapbec_amd import tbec, autotune
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
     c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
-    tile, seg = autotune.lookup(pr.shape, prec)
+    tile, seg = autotune.lookup(pr.shape, prec, sm)
     if args.tile >= 0:
         tile = args.tile
     if args.seg > 0:
@@ -142,6 +145,31 @@ def cpu_baseline(n):
                                       f"{cores} OpenMP threads", "s_per_step": dt}
 
 
+def pmc_probe(args):
+    """Kernels for counter runs: a copy of 2*n^3 complex64 (read 8 n^3 B... see below) and
+    two real-time steps (6 stage launches) of the FP32 stepper on a constant state."""
+    import numpy as np
+    import torch
+    from trapbec_amd import initial, tbec, autotune
+    n = args.n
+    src = torch.ones(n ** 3, dtype=torch.complex64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)          # reads and writes n^3 * 8 bytes
+    torch.cuda.synchronize()
+    del src, dst
+    pr = initial.Problem((n, n, n), (32.0, 32.0, 32.0), (1.0, 1.0, 1.0), 500.0, 0.0, 5e-4)
+    c = tbec.Condensate.from_problem(pr, prec=32, sum_mode=SUMS[args.fp32_sum])
+    c.set_tuning(*autotune.lookup(pr.shape, 32, SUMS[args.fp32_sum]))
+    psi = torch.full((n ** 3,), 0.1 + 0.05j, dtype=torch.complex64, device="cuda:0")
+    c.upload_device(psi.data_ptr())
+    c.sync()
+    c.step(2, tbec.REAL_RK3)
+    c.sync()
+    print(json.dumps({"pmc_probe": n, "copy_bytes": n ** 3 * 8,
+                      "stage_bytes_alg_avg": n ** 3 * 80 / 3}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -154,6 +182,9 @@ def main():
     from trapbec_amd import initial
 
     n = args.n
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
     print(f"[bench] building {n}^3 workload", file=sys.stderr, flush=True)
     pr, psi0 = initial.config_m(n)
     res = {}

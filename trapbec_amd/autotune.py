@@ -17,8 +17,10 @@ DEFAULT = {32: (0, 64), 64: (0, 64)}
 SEGS = (16, 32, 64, 128)
 
 
-def _key(shape, prec):
-    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}"
+def _key(shape, prec, sum_mode=None):
+    if sum_mode is None:
+        sum_mode = 0 if prec == 64 else 1
+    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}:s{sum_mode}"
 
 
 def _load():
@@ -28,19 +30,22 @@ def _load():
     return {}
 
 
-def lookup(shape, prec):
-    t = _load().get(_key(shape, prec))
+def lookup(shape, prec, sum_mode=None):
+    t = _load().get(_key(shape, prec, sum_mode))
     if t:
         return int(t["tile"]), int(t["seg"])
     return DEFAULT[prec]
 
 
-def tune(shape, prec, psi0=None, nstages=30, save=True, verbose=False):
-    """Time every (tile, seg) for this grid and precision; return the table entry."""
+def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
+         table=None):
+    """Time every (tile, seg) for this grid, precision and summation; return the entry."""
     import numpy as np
     from trapbec_amd import tbec
+    if sum_mode is None:
+        sum_mode = 0 if prec == 64 else 1
     c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
-                        dt=1e-4, prec=prec)
+                        dt=1e-4, prec=prec, sum_mode=sum_mode)
     if psi0 is None:
         psi0 = np.full(shape, 0.1 + 0.05j)
     c.upload(psi0)
@@ -58,8 +63,21 @@ def tune(shape, prec, psi0=None, nstages=30, save=True, verbose=False):
     entry = {"tile": tile, "seg": seg, "us_per_stage": ms * 1e3,
              "all": [[t, s, round(m * 1e3, 2)] for m, t, s in sorted(results)]}
     if save:
-        table = _load()
-        table[_key(shape, prec)] = entry
-        with open(TABLE, "w") as f:
-            json.dump(table, f, indent=1, sort_keys=True)
+        path = table or TABLE
+        t = _load() if path == TABLE else (json.load(open(path)) if os.path.exists(path) else {})
+        t[_key(shape, prec, sum_mode)] = entry
+        with open(path, "w") as f:
+            json.dump(t, f, indent=1, sort_keys=True)
     return entry
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time every tile shape and x-segment length")
+    ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 512])
+    ap.add_argument("--prec", type=int, default=32)
+    ap.add_argument("--sum", type=int, default=None)
+    ap.add_argument("--table", default=None, help="write here instead of tuning.json")
+    a = ap.parse_args()
+    e = tune(tuple(a.shape), a.prec, a.sum, verbose=True, table=a.table)
+    print(json.dumps({"shape": a.shape, "prec": a.prec, "sum": a.sum, **e}))

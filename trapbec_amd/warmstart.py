@@ -1,0 +1,166 @@
+"""Learned warm start for imaginary-time ground states.
+
+A small MLP maps trap, interaction, rotation and atom number (w_perp, w_z, g, Omega,
+log N) to a coarse 32^3 ground-state field (sqrt-density, cos phase, sin phase) on the
+box [-12, 12)^3.  It is trained on ground states the batched imaginary-time solver itself
+produces (Sobol-sampled parameters), on the GPU by libtbec's BF16-MFMA kernels.  Its
+prediction is trilinearly upsampled to the fine grid and refined by FP32 imaginary time;
+the plain Thomas-Fermi start is both the fallback and the judge.
+
+    python -m trapbec_amd.warmstart [--samples 2000] [--fine 384] [--out DIR]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from trapbec_amd import initial, tbec
+
+BOX = 24.0
+RANGES = np.array([[0.8, 1.2],      # w_perp
+                   [0.3, 1.5],      # w_z
+                   [200.0, 4000.0], # g
+                   [0.0, 0.8],      # Omega
+                   [math.log(0.5), math.log(2.0)]])  # log N
+COARSE = 32
+
+
+def sobol_params(n: int, seed: int) -> np.ndarray:
+    """n scrambled-Sobol points over RANGES (rows: w_perp, w_z, g, Omega, log N)."""
+    from scipy.stats import qmc
+    u = qmc.Sobol(d=5, scramble=True, seed=seed).random(n)
+    return RANGES[:, 0] + u * (RANGES[:, 1] - RANGES[:, 0])
+
+
+def features(params: np.ndarray) -> np.ndarray:
+    """Inputs scaled to [-1, 1]."""
+    return (2.0 * (params - RANGES[:, 0]) / (RANGES[:, 1] - RANGES[:, 0]) - 1.0).astype(
+        np.float32)
+
+
+def problem(p, n, dtau):
+    wp, wz, g, Om, logN = p
+    shape = (n, n, n)
+    return initial.Problem(shape, (BOX, BOX, BOX), (wp, wp, wz), g, Om, -dtau * 1j, False,
+                           {"norm": math.exp(logN)})
+
+
+def tf_start(pr, norm, seed=0, phase_amp=0.1):
+    psi, _ = initial.tf_profile(pr.shape, pr.box, pr.omega, pr.g, norm=norm)
+    return psi * initial.phase_noise(pr.shape, seed, phase_amp)
+
+
+def coarse_field(psi: np.ndarray, cn: int = COARSE) -> np.ndarray:
+    """(3, cn, cn, cn): block-averaged psi -> sqrt-density, cos phase, sin phase."""
+    n = psi.shape[0]
+    f = n // cn
+    blk = psi.reshape(cn, f, cn, f, cn, f).mean(axis=(1, 3, 5))
+    a = np.abs(blk)
+    ph = np.angle(blk)
+    return np.stack([a, np.cos(ph), np.sin(ph)]).astype(np.float32)
+
+
+def generate(n_samples, seed=11, n=64, dtau=5e-3, tol=1e-8, max_steps=8000, chunk=250,
+             log=print):
+    """Ground states for Sobol parameters via the batched FP32 imaginary-time solver."""
+    params = sobol_params(n_samples, seed)
+    X = features(params)
+    Y = np.zeros((n_samples, 3 * COARSE ** 3), np.float32)
+    stats = []
+    for c0 in range(0, n_samples, chunk):
+        idx = range(c0, min(n_samples, c0 + chunk))
+        # the ground state at atom number N and coupling g is sqrt(N) times the one at
+        # N = 1 and coupling g*N: every instance is solved at norm 1
+        probs = [fine_problem(params[i], n, dtau) for i in idx]
+        norms = [math.exp(params[i][4]) for i in idx]
+        t0 = time.time()
+        with tbec.Batch(probs, prec=32) as b:
+            for k, i in enumerate(idx):
+                b.upload(k, tf_start(probs[k], 1.0, seed=i))
+            res = b.imag_solve(max_steps, tol, 1.0, 100)
+            for k, i in enumerate(idx):
+                Y[i] = coarse_field(b.download(k) * math.sqrt(norms[k])).ravel()
+        stats += res
+        log(f"[warmstart] samples {c0}..{idx[-1]}: {time.time() - t0:.1f}s, "
+            f"converged {sum(r['converged'] for r in res)}/{len(res)}, "
+            f"mean steps {np.mean([r['steps'] for r in res]):.0f}")
+    return params, X, Y, stats
+
+
+def fine_problem(p, n, dtau):
+    """The refinement problem in units where the state is normalised to 1: g -> g N."""
+    wp, wz, g, Om, logN = p
+    return problem(np.array([wp, wz, g * math.exp(logN), Om, 0.0]), n, dtau)
+
+
+def refine(pr, start, tol, max_steps, check_every=200, coarse=None, prec=32):
+    """FP32 imaginary time to tol from a psi start (or a coarse field); returns stats."""
+    with tbec.Condensate.from_problem(pr, prec=prec) as c:
+        if coarse is not None:
+            tbec.upload_coarse(c, coarse)
+        else:
+            c.upload(start)
+        c.sync()
+        t0 = time.time()
+        r = c.imag_solve(max_steps, tol, 1.0, check_every)
+        c.sync()
+        r["wall_s"] = time.time() - t0
+        d = c.diagnose()
+        r["final_energy"], r["final_residual"] = d["energy"], d["residual"]
+        return r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--train-n", type=int, default=64)
+    ap.add_argument("--fine", type=int, default=384)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--fine-dtau", type=float, default=8e-4)
+    ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--out", default="gpurun_out/warmstart")
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+    t0 = time.time()
+    params, X, Y, stats = generate(a.samples, n=a.train_n)
+    t_gen = time.time() - t0
+    scale = np.ones(Y.shape[1], np.float32)
+    amp = Y[:, :COARSE ** 3].max()
+    scale[:COARSE ** 3] = 1.0 / amp          # sqrt-density channel to O(1)
+    with tbec.MLP(5, 512, 4, 3 * COARSE ** 3, seed=1) as m:
+        n_tr = int(0.9 * a.samples)
+        m.set_data(X[:n_tr], Y[:n_tr] * scale)
+        t1 = time.time()
+        hist = m.train(a.epochs, a.batch, a.lr, seed=2)
+        t_train = time.time() - t1
+        val_pred = m.forward(X[n_tr:]) / scale
+        val_mse = float(np.mean((val_pred - Y[n_tr:]) ** 2))
+        base_mse = float(np.mean((Y[:n_tr].mean(axis=0) - Y[n_tr:]) ** 2))
+        test = np.array([1.0, 0.8, 3000.0, 0.5, 0.0])
+        coarse = (m.forward(features(test[None]))[0] / scale).reshape(3, COARSE, COARSE, COARSE)
+    pr = fine_problem(test, a.fine, a.fine_dtau)
+    r_tf = refine(pr, tf_start(pr, 1.0, seed=99), a.tol, a.max_steps)
+    r_ml = refine(pr, None, a.tol, a.max_steps, coarse=coarse)
+    out = {"samples": a.samples, "gen_s": t_gen, "train_s": t_train,
+           "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+           "val_mse": val_mse, "val_mse_mean_predictor": base_mse,
+           "gen_converged": int(sum(s["converged"] for s in stats)),
+           "fine_grid": a.fine, "test_params": test.tolist(), "tf_start": r_tf,
+           "mlp_start": r_ml,
+           "speedup_steps": r_tf["steps"] / max(1, r_ml["steps"]),
+           "speedup_wall": r_tf["wall_s"] / max(1e-9, r_ml["wall_s"])}
+    with open(os.path.join(a.out, "warmstart.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
