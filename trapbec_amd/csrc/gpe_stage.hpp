@@ -54,6 +54,7 @@ struct StageArgs {
   int tiles_z, tiles_y, nseg;  // tile grid
   int nblocks;                 // tiles_z * tiles_y * nseg
   long long sy, sx, base;      // strides / interior offset (complex elements)
+  long long elems;             // allocated complex elements per buffer
   double x0, y0, z0;           // coordinate of interior index 0 on each axis
   double hx, hy, hz;
   double k2[3];                // 1 / (180 h_a^2)
@@ -70,6 +71,8 @@ struct StageArgs {
 };
 
 // ---- small helpers -------------------------------------------------------------
+__device__ __forceinline__ bool stage_reads_w(int st) { return st == ST_RK2 || st == ST_RK3; }
+
 template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
   s = a + b;
   T bb = s - a;
@@ -110,6 +113,15 @@ template <> struct VecIO<double, 2> {  // 2 complex doubles = 2 x 16 B
     q[1] = make_double2(o[1].re, o[1].im);
   }
 };
+template <> struct VecIO<float, 1> {
+  static __device__ __forceinline__ void ld(const cplx<float> *p, cplx<float> *o) {
+    float2 a = *reinterpret_cast<const float2 *>(p);
+    o[0] = {a.x, a.y};
+  }
+  static __device__ __forceinline__ void st(cplx<float> *p, const cplx<float> *o) {
+    *reinterpret_cast<float2 *>(p) = make_float2(o[0].re, o[0].im);
+  }
+};
 template <> struct VecIO<double, 1> {
   static __device__ __forceinline__ void ld(const cplx<double> *p, cplx<double> *o) {
     double2 a = *reinterpret_cast<const double2 *>(p);
@@ -117,6 +129,50 @@ template <> struct VecIO<double, 1> {
   }
   static __device__ __forceinline__ void st(cplx<double> *p, const cplx<double> *o) {
     *reinterpret_cast<double2 *>(p) = make_double2(o[0].re, o[0].im);
+  }
+};
+
+// Raw buffer access for the stage kernel: a uniform 64-bit plane base in a buffer
+// resource (SGPRs) plus a 32-bit per-lane byte offset (one VGPR), instead of a 64-bit
+// address per lane.  The resource's record count is the number of bytes left in the
+// allocation from that base, so every access is range-checked against it.
+typedef unsigned int tbec_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9-family raw buffer, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, long long bytes) {
+  const int n = (int)(bytes > 0x7fffffffLL ? 0x7fffffffLL : (bytes < 0 ? 0 : bytes));
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, n, kRsrcWord3);
+}
+
+template <typename T, int VZ> struct BufIO {
+  static constexpr int NB = VZ * 2 * (int)sizeof(T) / 16;   // 16-byte chunks
+  static __device__ __forceinline__ void ld(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                            cplx<T> *o) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const tbec_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * c, 0, 0);
+      if constexpr (sizeof(T) == 4) {
+        o[2 * c] = {__uint_as_float(v.x), __uint_as_float(v.y)};
+        o[2 * c + 1] = {__uint_as_float(v.z), __uint_as_float(v.w)};
+      } else {
+        o[c] = {__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z)};
+      }
+    }
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                            const cplx<T> *o) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      tbec_u32x4 v;
+      if constexpr (sizeof(T) == 4) {
+        v = tbec_u32x4{__float_as_uint(o[2 * c].re), __float_as_uint(o[2 * c].im),
+                       __float_as_uint(o[2 * c + 1].re), __float_as_uint(o[2 * c + 1].im)};
+      } else {
+        v = tbec_u32x4{(unsigned)__double2loint(o[c].re), (unsigned)__double2hiint(o[c].re),
+                       (unsigned)__double2loint(o[c].im), (unsigned)__double2hiint(o[c].im)};
+      }
+      __builtin_amdgcn_raw_buffer_store_b128(v, r, off + 16 * c, 0, 0);
+    }
   }
 };
 
@@ -201,10 +257,14 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const int tz = tid % NTZ, ty = tid / NTZ;
   const int z = zb + tz * VZ, y = yb + ty;
 
-  const C *__restrict__ pin = reinterpret_cast<const C *>(a.psi_in) + a.base;
-  C *__restrict__ pout = reinterpret_cast<C *>(a.psi_out) + a.base;
-  C *__restrict__ pw = reinterpret_cast<C *>(a.w) + a.base;
+  // Plane bases point at (row -3, column -4) of the interior so every per-thread offset
+  // within an x-plane is a non-negative 32-bit value; with a uniform 64-bit base that
+  // lets the loads and stores use the scalar-base + 32-bit-offset addressing form.
   const long long sx = a.sx, sy = a.sy;
+  const long long rb = a.base - 3 * sy - 4;
+  const C *__restrict__ pin = reinterpret_cast<const C *>(a.psi_in) + rb;
+  C *__restrict__ pout = reinterpret_cast<C *>(a.psi_out) + rb;
+  C *__restrict__ pw = reinterpret_cast<C *>(a.w) + rb;
 
   T sc = T(1);
   T c0 = T(a.c0);
@@ -213,12 +273,12 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     if (a.c0dev) c0 = T(*a.c0dev);
   }
 
-  // Own column pointer (x index added per plane).  Rows/columns beyond the interior
-  // read zeros from the padding; their results are never stored.
-  const long long own = (long long)y * sy + z;
+  // Own column offset within a plane.  Rows/columns beyond the interior read zeros from
+  // the padding; their results are never stored.
+  const uint32_t own = (uint32_t)(y + 3) * (uint32_t)sy + (uint32_t)(z + 4);
 
   // Halo assignment: vectors [0, 6*NTZ) are y-halo rows, the rest z-halo columns.
-  long long hoff = 0;
+  uint32_t hoff = 0;
   int hrow = 0, hcol = 0;
   const bool hl = tid < NHALO;
   if (hl) {
@@ -227,7 +287,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
       const int dy = r < 3 ? r - 3 : TY + r - 3;
       hrow = 3 + dy;
       hcol = 4 + cz * VZ;
-      hoff = (long long)(yb + dy) * sy + (zb + cz * VZ);
+      hoff = (uint32_t)(yb + dy + 3) * (uint32_t)sy + (uint32_t)(zb + cz * VZ + 4);
     } else {
       const int q = tid - 6 * NTZ;            // [0, 2*TY*HZV)
       const int side = q / (TY * HZV);
@@ -236,7 +296,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
       hrow = 3 + row;
       const int zc = side == 0 ? (-4 + v * VZ) : (TZ + v * VZ);
       hcol = 4 + zc;
-      hoff = (long long)(yb + row) * sy + (zb + zc);
+      hoff = (uint32_t)(yb + row + 3) * (uint32_t)sy + (uint32_t)(zb + zc + 4);
     }
   }
 
@@ -245,16 +305,31 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // when the queue advances) and the LDS buffer / halo register parity is static too.
   C q[8][VZ];
   C hb[2][VZ];
-  auto ldv = [&](const C *p, C *o) {
-    VecIO<T, VZ>::ld(p, o);
+  // buffer resources for x-plane `x` of the three arrays (bytes left in the allocation)
+  const long long left0 = (a.elems - rb) * (long long)sizeof(C);
+  auto rs_in = [&](int x) {
+    return plane_rsrc(pin + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
+  };
+  auto rs_w = [&](int x) {
+    return plane_rsrc(pw + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
+  };
+  auto rs_out = [&](int x) {
+    return plane_rsrc(pout + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
+  };
+  const uint32_t ownb = own * (uint32_t)sizeof(C), hoffb = hoff * (uint32_t)sizeof(C);
+  auto ldv = [&](__amdgpu_buffer_rsrc_t r, uint32_t off, C *o) {
+    BufIO<T, VZ>::ld(r, off, o);
     if constexpr (IMAG) {
 #pragma unroll
       for (int v = 0; v < VZ; ++v) { o[v].re *= sc; o[v].im *= sc; }
     }
   };
 #pragma unroll
-  for (int k = 0; k < 7; ++k) ldv(pin + (long long)(xs - 3 + k) * sx + own, q[k]);
-  if (hl) ldv(pin + (long long)xs * sx + hoff, hb[0]);
+  for (int k = 0; k < 7; ++k) ldv(rs_in(xs - 3 + k), ownb, q[k]);
+  if (hl) ldv(rs_in(xs), hoffb, hb[0]);
+  // w of the first plane (stages that read w); later planes are prefetched one ahead
+  C wq[2][VZ];
+  if (stage_reads_w(a.stage)) BufIO<T, VZ>::ld(rs_w(xs), ownb, wq[0]);
 
   const T k2x = T(a.k2[0]), k2y = T(a.k2[1]), k2z = T(a.k2[2]);
   const T k1x = T(a.k1[0]), k1y = T(a.k1[1]);
@@ -283,10 +358,10 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     constexpr int R = decltype(Rc)::value;   // (x - xs) & 7
     constexpr int buf = R & 1;
     // prefetch plane x+4 (own) and the halo of plane x+1
-    ldv(pin + (long long)(x + 4) * sx + own, q[(R + 7) & 7]);
-    if (hl) ldv(pin + (long long)(x + 1) * sx + hoff, hb[buf ^ 1]);
-    C wv[VZ];
-    if (stage == ST_RK2 || stage == ST_RK3) VecIO<T, VZ>::ld(pw + (long long)x * sx + own, wv);
+    ldv(rs_in(x + 4), ownb, q[(R + 7) & 7]);
+    if (hl) ldv(rs_in(x + 1), hoffb, hb[buf ^ 1]);
+    if (stage_reads_w(stage) && x + 1 < xe) BufIO<T, VZ>::ld(rs_w(x + 1), ownb, wq[buf ^ 1]);
+    const C *wv = wq[buf];
 
     // stage centre plane + halo into LDS
     {
@@ -375,23 +450,24 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     }
     // stores (interior points only; padding stays zero)
     if (yin) {
-      const long long o = (long long)x * sx + own;
+      C *po = pout + (long long)x * sx;   // uniform plane bases
+      C *pwx = pw + (long long)x * sx;
       if (full) {
         if (stage == ST_RHS) {
-          VecIO<T, VZ>::st(pw + o, wo);
+          BufIO<T, VZ>::st(rs_w(x), ownb, wo);
         } else {
-          VecIO<T, VZ>::st(pout + o, out);
-          if (stage == ST_RK1 || stage == ST_RK2) VecIO<T, VZ>::st(pw + o, wo);
+          BufIO<T, VZ>::st(rs_out(x), ownb, out);
+          if (stage == ST_RK1 || stage == ST_RK2) BufIO<T, VZ>::st(rs_w(x), ownb, wo);
         }
       } else {
 #pragma unroll
         for (int v = 0; v < VZ; ++v) {
           if (!zin[v]) continue;
           if (stage == ST_RHS) {
-            pw[o + v] = wo[v];
+            pwx[own + v] = wo[v];
           } else {
-            pout[o + v] = out[v];
-            if (stage == ST_RK1 || stage == ST_RK2) pw[o + v] = wo[v];
+            po[own + v] = out[v];
+            if (stage == ST_RK1 || stage == ST_RK2) pwx[own + v] = wo[v];
           }
         }
       }

@@ -11,13 +11,13 @@ namespace tbec {
 // Tile shapes (TZ, TY, VZ) per precision; the id is the index here.
 template <int PREC, int ID> struct TileShape;
 template <> struct TileShape<32, 0> { static constexpr int TZ = 64, TY = 16, VZ = 4; };
-template <> struct TileShape<32, 1> { static constexpr int TZ = 64, TY = 32, VZ = 4; };
-template <> struct TileShape<32, 2> { static constexpr int TZ = 128, TY = 8, VZ = 4; };
+template <> struct TileShape<32, 1> { static constexpr int TZ = 64, TY = 32, VZ = 2; };
+template <> struct TileShape<32, 2> { static constexpr int TZ = 64, TY = 16, VZ = 2; };
 template <> struct TileShape<32, 3> { static constexpr int TZ = 32, TY = 32, VZ = 4; };
 template <> struct TileShape<64, 0> { static constexpr int TZ = 32, TY = 16, VZ = 2; };
 template <> struct TileShape<64, 1> { static constexpr int TZ = 64, TY = 8, VZ = 2; };
-template <> struct TileShape<64, 2> { static constexpr int TZ = 32, TY = 32, VZ = 2; };
-template <> struct TileShape<64, 3> { static constexpr int TZ = 16, TY = 16, VZ = 2; };
+template <> struct TileShape<64, 2> { static constexpr int TZ = 32, TY = 16, VZ = 1; };
+template <> struct TileShape<64, 3> { static constexpr int TZ = 64, TY = 16, VZ = 1; };
 
 #define TBEC_DECL_LAUNCHER(P, I) \
   hipError_t launch_stage_p##P##_t##I(const StageArgs &a, int sum, bool rot, bool imag, \

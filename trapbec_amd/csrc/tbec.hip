@@ -52,8 +52,8 @@ int set_error(int code, const std::string &msg) {
 // ---------------------------------------------------------------- tile table ----
 // Each precision has a small set of tile shapes (TZ x TY x VZ); the segment length SX
 // is a runtime parameter.  The autotuner (trapbec_amd/autotune.py) picks (tile, SX).
-const TileCfg kTilesF32[] = {{64, 16, 4}, {64, 32, 4}, {128, 8, 4}, {32, 32, 4}};
-const TileCfg kTilesF64[] = {{32, 16, 2}, {64, 8, 2}, {32, 32, 2}, {16, 16, 2}};
+const TileCfg kTilesF32[] = {{64, 16, 4}, {64, 32, 2}, {64, 16, 2}, {32, 32, 4}};
+const TileCfg kTilesF64[] = {{32, 16, 2}, {64, 8, 2}, {32, 16, 1}, {64, 16, 1}};
 const int kNumTiles = 4;
 
 const TileCfg &tile_cfg(int prec, int id) {
@@ -225,7 +225,7 @@ int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, S
   a.tiles_y = (int)((p.ny + tc.ty - 1) / tc.ty);
   a.nseg = (p.nx + seg - 1) / seg;
   a.nblocks = a.tiles_z * a.tiles_y * a.nseg;
-  a.sy = L.sy; a.sx = L.sx; a.base = L.base;
+  a.sy = L.sy; a.sx = L.sx; a.base = L.base; a.elems = L.elems;
   a.x0 = -0.5 * p.Lx + 0.5 * L.hx;
   a.y0 = -0.5 * p.Ly + 0.5 * L.hy;
   a.z0 = -0.5 * p.Lz + 0.5 * L.hz;
