@@ -78,12 +78,13 @@ def run_gpu(args, prec, steps, psi_dev, pr, sum_name=None):
     from trapbec_amd import tbec, autotune
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
     c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
-    tile, seg = autotune.lookup(pr.shape, prec, sm)
+    tile, seg, remap = autotune.lookup(pr.shape, prec, sm)
     if args.tile >= 0:
         tile = args.tile
     if args.seg > 0:
         seg = args.seg
     c.set_tuning(tile, seg)
+    c.set_xcd_remap(remap)
     c.upload_device(psi_dev.data_ptr())
     c.sync()
     ms, wall = time_steps(torch, c, steps, args.warmup)
@@ -160,7 +161,9 @@ def pmc_probe(args):
     del src, dst
     pr = initial.Problem((n, n, n), (32.0, 32.0, 32.0), (1.0, 1.0, 1.0), 500.0, 0.0, 5e-4)
     c = tbec.Condensate.from_problem(pr, prec=32, sum_mode=SUMS[args.fp32_sum])
-    c.set_tuning(*autotune.lookup(pr.shape, 32, SUMS[args.fp32_sum]))
+    tile, seg, remap = autotune.lookup(pr.shape, 32, SUMS[args.fp32_sum])
+    c.set_tuning(tile, seg)
+    c.set_xcd_remap(remap)
     psi = torch.full((n ** 3,), 0.1 + 0.05j, dtype=torch.complex64, device="cuda:0")
     c.upload_device(psi.data_ptr())
     c.sync()

@@ -133,6 +133,9 @@ int tbec_set_norm(tbec_handle *h, double norm);
 /* Tile shape id (0..3, per precision) and x-segment length used by the stage kernel;
  * the autotuner (trapbec_amd/autotune.py) chooses them.  Drops captured graphs. */
 int tbec_set_tuning(tbec_handle *h, int tile, int seg);
+/* Tile order: 1 (default) gives each of the 8 XCDs a contiguous block of tiles so that
+ * neighbouring tiles share that XCD's L2; 0 keeps the dispatch order. */
+int tbec_set_xcd_remap(tbec_handle *h, int on);
 
 int tbec_sync(tbec_handle *h);
 /* The handle's HIP stream (hipStream_t), for event timing by the caller. */

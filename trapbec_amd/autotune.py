@@ -13,8 +13,8 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {32: (0, 64), 64: (0, 64)}
-SEGS = (16, 32, 64, 128)
+DEFAULT = {32: (2, 128), 64: (2, 128)}
+SEGS = (32, 64, 128, 256)
 
 
 def _key(shape, prec, sum_mode=None):
@@ -31,10 +31,11 @@ def _load():
 
 
 def lookup(shape, prec, sum_mode=None):
+    """(tile, seg, remap) recorded for this grid, precision and summation."""
     t = _load().get(_key(shape, prec, sum_mode))
     if t:
-        return int(t["tile"]), int(t["seg"])
-    return DEFAULT[prec]
+        return int(t["tile"]), int(t["seg"]), int(t.get("remap", 1))
+    return DEFAULT[prec] + (1,)
 
 
 def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
@@ -50,18 +51,21 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
         psi0 = np.full(shape, 0.1 + 0.05j)
     c.upload(psi0)
     results = []
-    for tile in range(4):
-        for seg in SEGS:
-            c.set_tuning(tile, seg)
-            c.time_stages(3)
-            ms = c.time_stages(nstages)
-            results.append((ms, tile, seg))
-            if verbose:
-                print(f"tile {tile} seg {seg}: {ms * 1e3:.1f} us/stage", flush=True)
+    for remap in (1, 0):
+        c.set_xcd_remap(remap)
+        for tile in range(4):
+            for seg in SEGS:
+                c.set_tuning(tile, seg)
+                c.time_stages(3)
+                ms = c.time_stages(nstages)
+                results.append((ms, tile, seg, remap))
+                if verbose:
+                    print(f"remap {remap} tile {tile} seg {seg}: {ms * 1e3:.1f} us/stage",
+                          flush=True)
     c.close()
-    ms, tile, seg = min(results)
-    entry = {"tile": tile, "seg": seg, "us_per_stage": ms * 1e3,
-             "all": [[t, s, round(m * 1e3, 2)] for m, t, s in sorted(results)]}
+    ms, tile, seg, remap = min(results)
+    entry = {"tile": tile, "seg": seg, "remap": remap, "us_per_stage": ms * 1e3,
+             "all": [[t, s, r, round(m * 1e3, 2)] for m, t, s, r in sorted(results)]}
     if save:
         path = table or TABLE
         t = _load() if path == TABLE else (json.load(open(path)) if os.path.exists(path) else {})

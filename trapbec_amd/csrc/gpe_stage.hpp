@@ -64,6 +64,7 @@ struct StageArgs {
   double cre, cim;             // complex update coefficient
   double A, B;                 // stage coefficients
   int stage;                   // StageKind
+  int remap;                   // 1: spread tiles over XCDs in contiguous chunks
   const double *scale;         // imaginary time: psi is read as scale*psi (or null)
   const double *c0dev;         // imaginary time: c0 from device memory (or null)
   const int *skip;             // imaginary time: converged flag; skip the pass when set
@@ -506,7 +507,7 @@ gpe_stage(const StageArgs a) {
   const int nb = a.nblocks;
   const int b = blockIdx.x;
   int lb = b;
-  if ((nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
+  if (a.remap && (nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
   march_tile<T, TZ, TY, VZ, SUM, ROT, IMAG>(a, lb, red);
   if constexpr (IMAG) {

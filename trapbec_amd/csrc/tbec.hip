@@ -241,6 +241,7 @@ int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, S
   a.c0 = p.c0; a.gnl = p.gnl; a.om = p.omega_z;
   a.cre = a.cim = a.A = a.B = 0.0;
   a.stage = 0;
+  a.remap = 1;
   a.scale = nullptr; a.c0dev = nullptr; a.skip = nullptr; a.partials = nullptr;
   return a.nblocks;
 }
@@ -262,6 +263,7 @@ static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A,
   a.w = h->w;
   a.cre = cre; a.cim = cim; a.A = A; a.B = B;
   a.stage = stage;
+  a.remap = h->remap;
   return a.nblocks;
 }
 
@@ -563,6 +565,14 @@ int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
   tbec_drop_graphs(h);
   h->tile = tile;
   h->seg = seg;
+  return 0;
+}
+
+int tbec_set_xcd_remap(tbec_handle *h, int on) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  tbec_drop_graphs(h);
+  h->remap = on ? 1 : 0;
   return 0;
 }
 

@@ -48,6 +48,7 @@ extern const int kNumTiles;
 struct tbec_handle {
   tbec_params p{};
   int tile = 0, seg = 64;
+  int remap = 1;                        // XCD-aware tile order
   int csize = 8;
   double hx = 0, hy = 0, hz = 0;
   double dt_re = 0, dt_im = 0;          // dt*(1 - i*gamma)

@@ -60,7 +60,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
-           "tbec_upload_coarse"]
+           "tbec_upload_coarse", "tbec_set_xcd_remap"]
 
 _lib = None
 
@@ -86,6 +86,7 @@ def load(path: str = LIB_PATH):
                                   C.POINTER(TbecSolveResult)]
     L.tbec_set_norm.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
+    L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
     L.tbec_sync.argtypes = [H]
     L.tbec_stream.argtypes = [H]
     L.tbec_stream.restype = C.c_void_p
@@ -213,6 +214,9 @@ class Condensate:
 
     def set_tuning(self, tile: int, seg: int):
         _check(load().tbec_set_tuning(self._h, tile, seg))
+
+    def set_xcd_remap(self, on: bool):
+        _check(load().tbec_set_xcd_remap(self._h, int(bool(on))))
 
     def sync(self):
         _check(load().tbec_sync(self._h))
