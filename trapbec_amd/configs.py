@@ -1,0 +1,163 @@
+"""The workload configurations, run end to end on the GPU, with their reported figures.
+
+    python -m trapbec_amd.configs c2|c3|c4 [--out DIR] [--n N]
+
+c2: real time at 256^3 in FP64 and FP32 from the same state: psi relative L2 error,
+    norm / energy drift and vortex-core displacement of FP32 against FP64.
+c3: imaginary-time ground state in a rotating trap at 256^3, FP32 bulk with FP64
+    reductions against the FP64 run: energy error, vortex-lattice count.
+c4: 48 condensates of uneven grids to energy convergence: one batched solve against
+    solving them one at a time on the same GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from trapbec_amd import initial, tbec
+
+
+def _cores(c, dens_min, orientation=0):
+    return c.vortex_cores(dens_min, orientation)
+
+
+def _match_displacement(a, b, h):
+    """Mean / max distance (in units of x) from each core of a to the nearest core of b
+    with the same charge and z index; plaquette positions in grid units times h."""
+    if len(a) == 0 or len(b) == 0:
+        return None, None
+    d = []
+    for i, j, k, q in a:
+        sel = b[(b[:, 2] == k) & (b[:, 3] == q)]
+        if len(sel) == 0:
+            continue
+        dd = np.sqrt((sel[:, 0] - i) ** 2 + (sel[:, 1] - j) ** 2).min() * h
+        d.append(dd)
+    if not d:
+        return None, None
+    return float(np.mean(d)), float(np.max(d))
+
+
+def run_c2(n=256, steps=2000):
+    pr, psi0 = initial.config_c2(n=n)
+    out = {"config": "c2", "grid": n, "steps": steps}
+    finals, diags = {}, {}
+    for prec in (64, 32):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.upload(psi0)
+            d0 = c.diagnose()
+            t0 = time.time()
+            c.step(steps)
+            c.sync()
+            wall = time.time() - t0
+            d1 = c.diagnose()
+            finals[prec] = c.download().astype(np.complex128)
+            cores = _cores(c, 0.05 * float(np.max(np.abs(psi0)) ** 2))
+        diags[prec] = {"norm0": d0["norm"], "norm1": d1["norm"], "E0": d0["energy"],
+                       "E1": d1["energy"], "wall_s": wall, "n_cores_z": int(len(cores))}
+        finals[(prec, "cores")] = cores
+    ref = finals[64]
+    out["psi_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(finals[32] - ref) / np.linalg.norm(ref))
+    out["psi_rel_max_fp32_vs_fp64"] = float(np.abs(finals[32] - ref).max() / np.abs(ref).max())
+    for prec in (64, 32):
+        d = diags[prec]
+        d["norm_drift"] = abs(d["norm1"] - d["norm0"]) / d["norm0"]
+        d["energy_drift"] = abs(d["E1"] - d["E0"]) / abs(d["E0"])
+        out[f"fp{prec}"] = d
+    h = pr.box[0] / n
+    mean_d, max_d = _match_displacement(finals[(32, "cores")], finals[(64, "cores")], h)
+    out["vortex_core_displacement_mean"] = mean_d
+    out["vortex_core_displacement_max"] = max_d
+    out["grid_spacing"] = h
+    return out
+
+
+def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
+    pr, psi0 = initial.config_c3(n=n)
+    out = {"config": "c3", "grid": n, "tol": tol}
+    res = {}
+    for prec in (64, 32):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.upload(psi0)
+            t0 = time.time()
+            r = c.imag_solve(max_steps, tol, 1.0, check_every)
+            c.sync()
+            r["wall_s"] = time.time() - t0
+            d = c.diagnose()
+            r["final_energy"] = d["energy"]
+            r["final_residual"] = d["residual"]
+            psi = c.download()
+            dens = float(np.max(np.abs(psi)) ** 2)
+            cores = c.vortex_cores(0.05 * dens, 0)
+            mid = n // 2
+            r["vortex_count_midplane"] = int(np.sum(cores[:, 2] == mid)) if len(cores) else 0
+            r["vortex_charge_sum_midplane"] = int(np.sum(cores[cores[:, 2] == mid][:, 3])) \
+                if len(cores) else 0
+        res[prec] = r
+    out["fp64"], out["fp32"] = res[64], res[32]
+    out["energy_rel_error_fp32_vs_fp64"] = abs(res[32]["final_energy"] - res[64]["final_energy"]) \
+        / abs(res[64]["final_energy"])
+    return out
+
+
+def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
+    probs = initial.config_c4(n_inst=n_inst)
+    out = {"config": "c4", "instances": n_inst, "tol": tol, "max_steps": max_steps,
+           "total_points": int(sum(np.prod(p.shape) for p, _ in probs))}
+    t0 = time.time()
+    with tbec.Batch([p for p, _ in probs], prec=prec) as b:
+        for i, (_, psi0) in enumerate(probs):
+            b.upload(i, psi0)
+        t1 = time.time()
+        res_b = b.imag_solve(max_steps, tol, 1.0, check_every)
+        t_batch = time.time() - t1
+    out["batched"] = {"wall_s": t_batch, "setup_s": t1 - t0,
+                      "converged": int(sum(r["converged"] for r in res_b)),
+                      "steps": [int(r["steps"]) for r in res_b]}
+    seq = []
+    t_seq = 0.0
+    for pr, psi0 in probs:
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.upload(psi0)
+            c.sync()
+            t1 = time.time()
+            r = c.imag_solve(max_steps, tol, 1.0, check_every)
+            c.sync()
+            t_seq += time.time() - t1
+            seq.append(r)
+    out["sequential"] = {"wall_s": t_seq,
+                         "converged": int(sum(r["converged"] for r in seq)),
+                         "steps": [int(r["steps"]) for r in seq]}
+    out["steps_identical"] = [int(a["steps"]) for a in res_b] == [int(a["steps"]) for a in seq]
+    pt_steps = sum(int(np.prod(p.shape)) * int(r["steps"]) for (p, _), r in zip(probs, res_b))
+    out["batched_gpt_steps_per_s"] = pt_steps / t_batch / 1e9
+    out["sequential_gpt_steps_per_s"] = pt_steps / t_seq / 1e9
+    out["speedup"] = t_seq / t_batch
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", choices=["c2", "c3", "c4"])
+    ap.add_argument("--out", default="gpurun_out/configs")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--max-steps", type=int, default=None)
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+    if a.config == "c2":
+        r = run_c2(n=a.n)
+    elif a.config == "c3":
+        r = run_c3(n=a.n, max_steps=a.max_steps or 200000)
+    else:
+        r = run_c4(max_steps=a.max_steps or 20000)
+    with open(os.path.join(a.out, f"{a.config}.json"), "w") as f:
+        json.dump(r, f, indent=1)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
