@@ -1,7 +1,9 @@
 """Throughput of the fused RK3 GPE stepper on one MI355X.
 
 Workload: the 512^3 real-time GPE on [-16,16)^3 (h = 1/16), V = r^2/2, g = 500,
-psi0 = TF(mu = 10) x z-line vortex + 1e-3 seeded noise, dt = 5e-4, zero ghosts, Omega = 0.
+psi0 = TF(mu = 10) x z-line vortex + 1e-3 seeded noise (generated on the device by
+libtbec's seeded generator, equal to trapbec_amd/initial.py's), dt = 5e-4, zero ghosts,
+Omega = 0.
 A step is one low-storage RK3 step (3 fused stage launches).  psi is resident in HBM
 before the timed region; K steps are replayed as captured graphs and timed with HIP
 events on the stepper's own stream, bracketed by device synchronisation.
@@ -73,7 +75,7 @@ def time_steps(torch, c, steps, warmup):
 SUMS = {"reference": 0, "compensated": 1, "centred": 2}
 
 
-def run_gpu(args, prec, steps, psi_dev, pr, sum_name=None):
+def run_gpu(args, prec, steps, spec, pr, sum_name=None):
     import torch
     from trapbec_amd import tbec, autotune
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
@@ -85,7 +87,7 @@ def run_gpu(args, prec, steps, psi_dev, pr, sum_name=None):
         seg = args.seg
     c.set_tuning(tile, seg)
     c.set_xcd_remap(remap)
-    c.upload_device(psi_dev.data_ptr())
+    c.init_state(spec)            # seeded state generated in HBM (not timed)
     c.sync()
     ms, wall = time_steps(torch, c, steps, args.warmup)
     npts = pr.shape[0] * pr.shape[1] * pr.shape[2]
@@ -188,30 +190,24 @@ def main():
     if args.pmc_probe:
         pmc_probe(args)
         return
-    print(f"[bench] building {n}^3 workload", file=sys.stderr, flush=True)
-    pr, psi0 = initial.config_m(n)
+    pr, spec = initial.ic_spec("m", n)
     res = {}
     for prec in (32, 64):
         if args.only != "both" and args.only != f"f{prec}":
             continue
-        cdt = torch.complex64 if prec == 32 else torch.complex128
-        psi_dev = torch.from_numpy(psi0.astype(np.complex64 if prec == 32 else np.complex128)
-                                   ).to("cuda:0")
-        assert psi_dev.dtype == cdt and psi_dev.is_contiguous()
         steps = args.steps if prec == 32 else args.fp64_steps
         print(f"[bench] fp{prec}: {steps} timed steps", file=sys.stderr, flush=True)
-        res[prec] = run_gpu(args, prec, steps, psi_dev, pr)
+        res[prec] = run_gpu(args, prec, steps, spec, pr)
         print(f"[bench] fp{prec}: {res[prec]['ms_per_step']:.3f} ms/step "
               f"{res[prec]['gpt_steps_per_s']:.2f} Gpt-steps/s", file=sys.stderr, flush=True)
         if prec == 32 and args.variants:
             for name in SUMS:
                 if name == args.fp32_sum:
                     continue
-                v = run_gpu(args, 32, steps, psi_dev, pr, name)
+                v = run_gpu(args, 32, steps, spec, pr, name)
                 res.setdefault("variants", {})[name] = v
                 print(f"[bench] fp32 {name}: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
                       flush=True)
-        del psi_dev
         torch.cuda.empty_cache()
     head = 32 if 32 in res else 64
     r = res[head]

@@ -127,6 +127,25 @@ int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
 int tbec_vortex_cores(tbec_handle *h, double dens_min, int orientation, int32_t *out,
                       int64_t cap, int64_t *count);
 
+/* Seeded initial state generated on the device (no reference counterpart: the
+ * reference builds states on the host, src/initial.f90, with a clock-seeded RNG):
+ *   psi = scale * [ sqrt(max(mu - V, 0)/g) * ring * prod(lines) * exp(i theta) + noise ]
+ * V = sum omega_a^2 x_a^2 / 2; ring and lines are tanh(d/(sqrt(2) xi)) exp(i q phi) vortex
+ * factors; theta uniform in [-phase_noise_amp, phase_noise_amp]; noise complex Gaussian
+ * of amplitude noise_amp.  Random numbers are a counter-based function of
+ * (seed, stream, flat point index), identical to trapbec_amd/initial.py. */
+typedef struct {
+  double omega[3];
+  double g, mu, scale, xi;
+  double ring_radius, ring_z0;      /* no ring if ring_radius <= 0 */
+  double noise_amp, phase_noise_amp;
+  double line_x[8], line_y[8];
+  int32_t line_q[8];
+  int32_t n_lines;
+  uint32_t seed;
+} tbec_ic;
+int tbec_init_state(tbec_handle *h, const tbec_ic *ic);
+
 /* Set the norm target used by TBEC_IMAG_RK3 steps (default 1). */
 int tbec_set_norm(tbec_handle *h, double norm);
 

@@ -257,3 +257,30 @@ def config_c4(seed: int = 20261203, n_inst: int = 48, dx: float = 0.2, dtau: flo
         pr.meta.update(mu=mu)
         out.append((pr, psi))
     return out
+
+
+def ic_spec(name: str, n: int = None, seed: int = 20261203) -> tuple:
+    """(Problem, spec) of a configuration for the device generator (tbec_init_state);
+    the spec reproduces config_<name>'s numpy state."""
+    if name in ("m", "c1"):
+        n = n or (512 if name == "m" else 64)
+        box = (32.0,) * 3 if name == "m" else (16.0,) * 3
+        dt = 5e-4 if name == "m" else 1e-3
+        ref = name == "c1"
+        pr = Problem((n, n, n), box, (1.0, 1.0, 1.0), 500.0, 0.0, dt, ref,
+                     {"name": f"{name}{n}", "seed": seed})
+        spec = dict(omega=(1.0, 1.0, 1.0), g=500.0, mu=10.0, scale=pr.psi_scale(),
+                    xi=1.0 / math.sqrt(10.0), lines=[(0.0, 0.0, 1)], noise_amp=1e-3,
+                    seed=seed)
+        return pr, spec
+    if name == "c2":
+        pr, _ = config_c2(seed, n or 256)
+        spec = dict(omega=pr.omega, g=pr.g, mu=pr.meta["mu"], xi=1.0 / math.sqrt(2.0 * pr.meta["mu"]),
+                    ring_radius=4.0, ring_z0=0.0, lines=pr.meta["lines"], seed=seed)
+        return pr, spec
+    if name == "c3":
+        pr, _ = config_c3(seed, n or 256)
+        spec = dict(omega=pr.omega, g=pr.g, mu=pr.meta["mu"], xi=1.0, phase_noise_amp=0.1,
+                    seed=seed)
+        return pr, spec
+    raise ValueError(name)

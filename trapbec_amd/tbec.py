@@ -45,6 +45,15 @@ class TbecSolveResult(C.Structure):
                 ("norm", C.c_double), ("rel_change", C.c_double)]
 
 
+class TbecIc(C.Structure):
+    _fields_ = [("omega", C.c_double * 3), ("g", C.c_double), ("mu", C.c_double),
+                ("scale", C.c_double), ("xi", C.c_double), ("ring_radius", C.c_double),
+                ("ring_z0", C.c_double), ("noise_amp", C.c_double),
+                ("phase_noise_amp", C.c_double), ("line_x", C.c_double * 8),
+                ("line_y", C.c_double * 8), ("line_q", C.c_int32 * 8),
+                ("n_lines", C.c_int32), ("seed", C.c_uint32)]
+
+
 class TbecError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -60,7 +69,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
-           "tbec_upload_coarse", "tbec_set_xcd_remap"]
+           "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state"]
 
 _lib = None
 
@@ -87,6 +96,7 @@ def load(path: str = LIB_PATH):
     L.tbec_set_norm.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
+    L.tbec_init_state.argtypes = [H, C.POINTER(TbecIc)]
     L.tbec_sync.argtypes = [H]
     L.tbec_stream.argtypes = [H]
     L.tbec_stream.restype = C.c_void_p
@@ -171,6 +181,19 @@ class Condensate:
         out = np.empty(self.shape, dtype=self.cdt)
         _check(load().tbec_download_psi(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def init_state(self, spec: dict):
+        """Generate a seeded initial state on the device (see initial.ic_spec)."""
+        lines = spec.get("lines", [])
+        ic = TbecIc((C.c_double * 3)(*spec["omega"]), spec["g"], spec["mu"],
+                    spec.get("scale", 1.0), spec["xi"], spec.get("ring_radius", 0.0),
+                    spec.get("ring_z0", 0.0), spec.get("noise_amp", 0.0),
+                    spec.get("phase_noise_amp", 0.0),
+                    (C.c_double * 8)(*[l[0] for l in lines]),
+                    (C.c_double * 8)(*[l[1] for l in lines]),
+                    (C.c_int32 * 8)(*[int(l[2]) for l in lines]), len(lines),
+                    spec.get("seed", 0) & 0xFFFFFFFF)
+        _check(load().tbec_init_state(self._h, C.byref(ic)))
 
     def upload_device(self, ptr: int):
         _check(load().tbec_upload_psi_device(self._h, C.c_void_p(ptr)))
