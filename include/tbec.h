@@ -81,7 +81,7 @@ typedef struct {
 
 typedef struct {
   int64_t steps;        /* steps taken */
-  int32_t converged;    /* 1 if the relative energy change fell below tol */
+  int32_t converged;    /* 1 if the relative energy change stayed below tol for 8 steps */
   int32_t pad_;
   double energy, mu, residual, norm;
   double rel_change;    /* |E_n - E_{n-1}| / |E_n| at the last step */
@@ -113,8 +113,10 @@ int tbec_rhs(tbec_handle *h, void *rhs_host);
 int tbec_diagnose(tbec_handle *h, tbec_diag *out);
 
 /* Imaginary-time ground state: TBEC_IMAG_RK3 steps with dtau = -dt_im until
- * |E_n - E_{n-1}| < tol*|E_n| or max_steps; renormalises to `norm` every step.  The
- * host checks convergence every check_every steps (a captured graph in between). */
+ * |E_n - E_{n-1}| < tol*|E_n| has held for 8 consecutive steps, or max_steps;
+ * renormalises to `norm` every step.  The host checks convergence every check_every
+ * steps (a captured graph in between).  With TBEC_PROGRESS=<seconds> in the
+ * environment, a progress line goes to stderr at that interval. */
 int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
                     int check_every, tbec_solve_result *out);
 

@@ -142,23 +142,27 @@ class Stepper:
         lib().orc_sums_f64(C.byref(self.p), _ptr(self.psi), _ptr(self.rhs), _ptr(out))
         return out
 
-    def imag_rk3(self, nsteps: int, norm_target: float, tol: float = 0.0) -> list:
-        """Renormalised imaginary-time RK3 (dt must be (0, -dtau)).  Stops early when the
-        relative energy change between consecutive steps drops below tol.  Returns the
-        per-step diagnostics [N, E, mu, residual] of the state entering each step."""
+    def imag_rk3(self, nsteps: int, norm_target: float, tol: float = 0.0,
+                 hits: int = 8) -> list:
+        """Renormalised imaginary-time RK3 (dt must be (0, -dtau)).  Stops early once the
+        relative energy change between consecutive steps has stayed below tol for `hits`
+        steps in a row.  Returns the per-step diagnostics [N, E, mu, residual] of the
+        state entering each step."""
         assert self.prec == 64
         if not hasattr(self, "c0_eff"):
             self.c0_eff = np.array([self.p.c0])
         hist = []
         diag = np.zeros(4)
+        run = 0
         for _ in range(nsteps):
             lib().orc_imag_rk3_step_f64(C.byref(self.p), _ptr(self.psi), _ptr(self.w),
                                         _ptr(self.rhs), norm_target, _ptr(self.c0_eff),
                                         _ptr(diag))
             hist.append(diag.copy())
-            if tol > 0 and len(hist) > 1 and \
-                    abs(hist[-1][1] - hist[-2][1]) < tol * abs(hist[-1][1]):
-                break
+            if tol > 0 and len(hist) > 1:
+                run = run + 1 if abs(hist[-1][1] - hist[-2][1]) < tol * abs(hist[-1][1]) else 0
+                if run >= hits:
+                    break
         return hist
 
     def vortices(self, dens_min: float, orientation: int = 0, cap: int = 1 << 20) -> np.ndarray:

@@ -167,6 +167,33 @@ def test_imag_rk3_renorm_vs_oracle():
     assert len(hist) == 40
 
 
+def test_imag_solve_stop_step_vs_oracle():
+    """tbec_imag_solve stops at the step where the oracle's stop rule (relative energy
+    change below tol for 8 steps in a row) fires and returns that step's input psi_m,
+    with the same energy."""
+    shape, box, om, g = (24, 24, 24), (12.0, 12.0, 12.0), (1.0, 1.1, 0.9), 20.0
+    announce(f"imag solve stop step {shape}")
+    pr = initial.Problem(shape, box, om, g, 0.3, -5e-3j)
+    kp = pr.kernel_params()
+    psi0 = initial.tf_profile(shape, box, om, g, norm=1.0)[0] * \
+        initial.phase_noise(shape, 5, 0.1)
+    s = oracle.Stepper(oracle.params(shape, box, kp["harm_w"], omega_z=kp["omega_r"],
+                                     c0=kp["c0"], gnl=kp["gnl"], dt=kp["dt"]), psi0)
+    hist = s.imag_rk3(5000, 1.0, tol=1e-7)
+    m = len(hist) - 1        # the oracle also took the step out of the converged psi_m
+    assert 8 < m < 4999
+    s2 = oracle.Stepper(s.p, psi0)
+    s2.imag_rk3(m, 1.0)
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        r = c.imag_solve(5000, 1e-7, 1.0, 50)
+        got = c.download()
+    assert r["converged"] == 1
+    assert r["steps"] == m
+    assert r["energy"] == pytest.approx(hist[-1][1], rel=1e-12)
+    assert relmax(got, s2.interior) < 1e-11
+
+
 def test_imag_solve_converges_harmonic():
     """g = 0 isotropic trap from a displaced Gaussian: converges to E = 3/2 (FP64)."""
     shape, box = (32, 32, 32), (12.0, 12.0, 12.0)

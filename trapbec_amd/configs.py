@@ -16,6 +16,8 @@ import json
 import os
 import time
 
+os.environ.setdefault("TBEC_PROGRESS", "20")
+
 import numpy as np
 
 from trapbec_amd import initial, tbec
@@ -56,7 +58,7 @@ def run_c2(n=256, steps=2000):
             wall = time.time() - t0
             d1 = c.diagnose()
             finals[prec] = c.download().astype(np.complex128)
-            cores = _cores(c, 0.05 * float(np.max(np.abs(psi0)) ** 2))
+            cores = _cores(c, 1e-3 * float(np.max(np.abs(psi0)) ** 2))
         diags[prec] = {"norm0": d0["norm"], "norm1": d1["norm"], "E0": d0["energy"],
                        "E1": d1["energy"], "wall_s": wall, "n_cores_z": int(len(cores))}
         finals[(prec, "cores")] = cores
@@ -83,6 +85,7 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
     for prec in (64, 32):
         with tbec.Condensate.from_problem(pr, prec=prec) as c:
             c.upload(psi0)
+            print(f"[c3] fp{prec} imaginary-time solve {pr.shape}", flush=True)
             t0 = time.time()
             r = c.imag_solve(max_steps, tol, 1.0, check_every)
             c.sync()
@@ -92,7 +95,7 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
             r["final_residual"] = d["residual"]
             psi = c.download()
             dens = float(np.max(np.abs(psi)) ** 2)
-            cores = c.vortex_cores(0.05 * dens, 0)
+            cores = c.vortex_cores(1e-3 * dens, 0)
             mid = n // 2
             r["vortex_count_midplane"] = int(np.sum(cores[:, 2] == mid)) if len(cores) else 0
             r["vortex_charge_sum_midplane"] = int(np.sum(cores[cores[:, 2] == mid][:, 3])) \
@@ -118,9 +121,11 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
     out["batched"] = {"wall_s": t_batch, "setup_s": t1 - t0,
                       "converged": int(sum(r["converged"] for r in res_b)),
                       "steps": [int(r["steps"]) for r in res_b]}
+    print(f"[c4] batched: {t_batch:.1f} s, {out['batched']['converged']}/{n_inst} converged",
+          flush=True)
     seq = []
     t_seq = 0.0
-    for pr, psi0 in probs:
+    for k, (pr, psi0) in enumerate(probs):
         with tbec.Condensate.from_problem(pr, prec=prec) as c:
             c.upload(psi0)
             c.sync()
@@ -129,6 +134,8 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
             c.sync()
             t_seq += time.time() - t1
             seq.append(r)
+        print(f"[c4] sequential {k + 1}/{n_inst} {pr.shape}: {int(r['steps'])} steps, "
+              f"{t_seq:.1f} s so far", flush=True)
     out["sequential"] = {"wall_s": t_seq,
                          "converged": int(sum(r["converged"] for r in seq)),
                          "steps": [int(r["steps"]) for r in seq]}

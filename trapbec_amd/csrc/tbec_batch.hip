@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -403,6 +405,8 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
   long long chunk = check_every + (check_every & 1);
   long long done = 0;
   bool all = false;
+  const double every = progress_interval();
+  auto last = std::chrono::steady_clock::now();
   while (done < max_steps && !all) {
     const long long nst = std::min<long long>(chunk, max_steps - done);
     rc = nst == chunk && chunk > 1 ? replay_batch(b, nst) : enqueue_batch_steps(b, nst);
@@ -412,7 +416,19 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
                         hipMemcpyDeviceToHost, b->stream));
     BCHK(hipStreamSynchronize(b->stream));
     all = true;
-    for (int i = 0; i < b->n; ++i) all = all && b->h_flags[4 * i] != 0;
+    int nconv = 0;
+    for (int i = 0; i < b->n; ++i) {
+      all = all && b->h_flags[4 * i] != 0;
+      nconv += b->h_flags[4 * i] != 0;
+    }
+    if (every > 0) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last).count() >= every) {
+        std::fprintf(stderr, "[tbec] batch solve: %lld steps, %d/%d finished\n", done, nconv,
+                     b->n);
+        last = now;
+      }
+    }
   }
   BCHK(hipMemcpyAsync(b->h_state, b->d_state, sizeof(double) * ST_COUNT * b->n,
                       hipMemcpyDeviceToHost, b->stream));

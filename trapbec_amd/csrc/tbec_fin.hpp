@@ -74,7 +74,8 @@ __device__ inline void fin_energy(const double s[4], double *st, int *flag, int 
   st[ST_REL] = rel;
   st[ST_EPREV] = E;
   st[ST_C0NEXT] = st[ST_C0BASE] + 2.0 * mu;
-  if (rel < st[ST_TOL]) {
+  st[ST_HITS] = rel < st[ST_TOL] ? st[ST_HITS] + 1.0 : 0.0;
+  if (st[ST_HITS] >= (double)kConvergeHits) {
     flag[0] = 1;
     flag[1] = (int)st[ST_STEP];
   }

@@ -24,6 +24,7 @@ enum StateSlot {
   ST_EPREV,
   ST_N, ST_E, ST_MU, ST_RES, ST_REL,
   ST_MAXA, ST_MINA, ST_SUMA, ST_SUMA2,
+  ST_HITS,        // consecutive steps below the tolerance
   ST_COUNT
 };
 
@@ -42,6 +43,14 @@ bool rot_enabled(const tbec_params &p);
 long long round_up(long long v, long long m);
 const TileCfg &tile_cfg(int prec, int id);
 extern const int kNumTiles;
+
+// A solve has converged after this many consecutive steps with a relative energy
+// change below the tolerance (one step can dip below it by rounding noise in FP32).
+constexpr int kConvergeHits = 8;
+
+// Progress lines on stderr for long solves when TBEC_PROGRESS is set (seconds between
+// lines, default 10).
+double progress_interval();
 
 }  // namespace tbec
 

@@ -13,6 +13,16 @@
 
 using namespace tbec;
 
+#include <chrono>
+#include <cstdlib>
+
+double tbec::progress_interval() {
+  const char *e = std::getenv("TBEC_PROGRESS");
+  if (!e) return -1.0;
+  const double v = std::atof(e);
+  return v > 0 ? v : 10.0;
+}
+
 #define HIPCHK2(x)                                                                     \
   do {                                                                                 \
     hipError_t e_ = (x);                                                               \
@@ -154,6 +164,8 @@ int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
   if (chunk & 1) ++chunk;
   long long done = 0;
   int flag = 0;
+  const double every = progress_interval();
+  auto last = std::chrono::steady_clock::now();
   while (done < max_steps) {
     long long n = std::min<long long>(chunk, max_steps - done);
     if (n == chunk && chunk > 1) rc = replay_chunk(h, n, TBEC_IMAG_RK3);
@@ -165,6 +177,13 @@ int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
     HIPCHK2(hipStreamSynchronize(h->stream));
     flag = h->hflag[0];
     if (flag) break;
+    if (every > 0) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last).count() >= every) {
+        std::fprintf(stderr, "[tbec] imaginary-time solve: %lld steps\n", done);
+        last = now;
+      }
+    }
   }
   HIPCHK2(hipMemcpyAsync(h->hstate, h->dstate, sizeof(double) * ST_COUNT,
                          hipMemcpyDeviceToHost, h->stream));

@@ -19,7 +19,9 @@ import time
 
 import numpy as np
 
-from trapbec_amd import initial, tbec
+os.environ.setdefault("TBEC_PROGRESS", "20")
+
+from trapbec_amd import initial, tbec  # noqa: E402
 
 BOX = 24.0
 RANGES = np.array([[0.8, 1.2],      # w_perp
