@@ -10,10 +10,16 @@ events on the stepper's own stream, bracketed by device synchronisation.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu]
 
-Prints one JSON line.  `value` is FP32 (compensated stencil sums) Gpt-steps/s; the FP64
-figure is reported beside it.  Roofline: algorithmic bytes per grid-point step are 10
-complex transfers (80 B FP32, 160 B FP64): stage 1 reads psi, writes w and psi1; stage 2
-reads psi1 and w, writes w and psi2; stage 3 reads psi2 and w, writes psi.
+Prints one JSON line.  `value` is FP32 Gpt-steps/s with compensated stencil sums in
+the centred-difference form and w rebuilt from the psi pair; FP64 and the stored-w
+scheme are reported beside it (--variants adds the other FP32 summation forms).
+
+Roofline basis: the reference's RK3 scheme moves 10 complex values per grid-point step
+(80 B FP32, 160 B FP64): stage 1 reads psi, writes w and psi1; stage 2 reads psi1 and w,
+writes w and psi2; stage 3 reads psi2 and w, writes psi.  `roofline.achieved` and
+`frac` use that basis for every variant.  With w rebuilt (stage 2 reads psi1 and psi0,
+stage 3 reads psi2 and psi1, no w) the kernels need only 8 (64 B / 128 B);
+`roofline.frac_of_bytes_moved` states the fraction on that basis.
 """
 from __future__ import annotations
 
@@ -29,7 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-BYTES_PER_PT_STEP = {32: 80, 64: 160}
+BYTES_PER_PT_STEP = {32: 80, 64: 160}          # reference scheme, 10 transfers
+MOVED_PER_PT_STEP = {(32, False): 64, (64, False): 128, (32, True): 80, (64, True): 160}
 
 
 def parse():
@@ -44,7 +51,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
     ap.add_argument("--fp32-sum", choices=["compensated", "centred", "reference"],
-                    default="compensated")
+                    default="centred")
     ap.add_argument("--variants", action="store_true",
                     help="also time the other FP32 summation modes")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -75,12 +82,13 @@ def time_steps(torch, c, steps, warmup):
 SUMS = {"reference": 0, "compensated": 1, "centred": 2}
 
 
-def run_gpu(args, prec, steps, spec, pr, sum_name=None):
+def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False):
     import torch
     from trapbec_amd import tbec, autotune
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
     c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
-    tile, seg, remap = autotune.lookup(pr.shape, prec, sm)
+    c.set_w_storage(w_stored)
+    tile, seg, remap = autotune.lookup(pr.shape, prec, sm, w_stored)
     if args.tile >= 0:
         tile = args.tile
     if args.seg > 0:
@@ -93,10 +101,11 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None):
     npts = pr.shape[0] * pr.shape[1] * pr.shape[2]
     gpts = npts / (ms * 1e-3) / 1e9
     achieved = npts * BYTES_PER_PT_STEP[prec] / (ms * 1e-3)
+    moved = npts * MOVED_PER_PT_STEP[(prec, w_stored)] / (ms * 1e-3)
     c.close()
     return {"ms_per_step": ms, "gpt_steps_per_s": gpts, "wall_s": wall,
-            "achieved_Bps": achieved, "tile": tile, "seg": seg,
-            "ms_per_launch": ms / 3.0}
+            "achieved_Bps": achieved, "moved_Bps": moved, "tile": tile, "seg": seg,
+            "remap": remap, "ms_per_launch": ms / 3.0, "w_stored": w_stored}
 
 
 def cpu_baseline(n):
@@ -149,8 +158,9 @@ def cpu_baseline(n):
 
 
 def pmc_probe(args):
-    """Kernels for counter runs: a copy of 2*n^3 complex64 (read 8 n^3 B... see below) and
-    two real-time steps (6 stage launches) of the FP32 stepper on a constant state."""
+    """Kernels for counter runs: a device copy of n^3 complex64 (reads and writes 8 n^3
+    bytes: the counter calibration) and two real-time steps (6 stage launches) of the
+    FP32 stepper on a constant state, with the headline settings."""
     import numpy as np
     import torch
     from trapbec_amd import initial, tbec, autotune
@@ -172,7 +182,8 @@ def pmc_probe(args):
     c.step(2, tbec.REAL_RK3)
     c.sync()
     print(json.dumps({"pmc_probe": n, "copy_bytes": n ** 3 * 8,
-                      "stage_bytes_alg_avg": n ** 3 * 80 / 3}), flush=True)
+                      "stage_bytes_reference_scheme_avg": n ** 3 * 80 / 3,
+                      "stage_bytes_moved_avg": n ** 3 * 64 / 3}), flush=True)
 
 
 def main():
@@ -200,6 +211,10 @@ def main():
         res[prec] = run_gpu(args, prec, steps, spec, pr)
         print(f"[bench] fp{prec}: {res[prec]['ms_per_step']:.3f} ms/step "
               f"{res[prec]['gpt_steps_per_s']:.2f} Gpt-steps/s", file=sys.stderr, flush=True)
+        v = run_gpu(args, prec, steps, spec, pr, None, True)
+        res.setdefault("stored", {})[prec] = v
+        print(f"[bench] fp{prec} stored w: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
         if prec == 32 and args.variants:
             for name in SUMS:
                 if name == args.fp32_sum:
@@ -227,26 +242,33 @@ def main():
         "config": {"workload": f"real-time GPE RK3, {n}^3, box [-16,16)^3, g=500, dt=5e-4, "
                                f"zero ghosts, Omega=0",
                    "grid": [n, n, n], "stencil": "6th-order FD, fused RK3 stage",
-                   "fp32_sum": args.fp32_sum, "tile": r["tile"], "seg": r["seg"]},
+                   "fp32_sum": args.fp32_sum, "w": "rebuilt", "tile": r["tile"],
+                   "seg": r["seg"], "xcd_remap": r["remap"]},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
                      "traffic": None,
                      "kernel": "gpe_stage (one RK3 stage)",
+                     "basis": "reference RK3 scheme: 10 complex transfers per point-step",
                      "bytes_per_launch": n ** 3 * BYTES_PER_PT_STEP[head] / 3,
-                     "ms_per_launch": round(r["ms_per_launch"], 5)},
+                     "ms_per_launch": round(r["ms_per_launch"], 5),
+                     "bytes_moved_per_launch": n ** 3 * MOVED_PER_PT_STEP[(head, False)] / 3,
+                     "frac_of_bytes_moved": round(r["moved_Bps"] / HBM_PEAK, 4)},
     }
+
+    def brief(v, steps):
+        return {"value": round(v["gpt_steps_per_s"], 4), "unit": "Gpt-steps/s",
+                "ms_per_step": round(v["ms_per_step"], 5),
+                "frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
+                "frac_of_bytes_moved": round(v["moved_Bps"] / HBM_PEAK, 4),
+                "achieved_GBps": round(v["achieved_Bps"] / 1e9, 1),
+                "tile": v["tile"], "seg": v["seg"], "steps": steps}
     if 64 in res and head == 32:
-        f = res[64]
-        out["fp64"] = {"value": round(f["gpt_steps_per_s"], 4), "unit": "Gpt-steps/s",
-                       "ms_per_step": round(f["ms_per_step"], 5),
-                       "frac": round(f["achieved_Bps"] / HBM_PEAK, 4),
-                       "achieved_GBps": round(f["achieved_Bps"] / 1e9, 1),
-                       "steps": args.fp64_steps}
+        out["fp64"] = brief(res[64], args.fp64_steps)
+    for prec, v in res.get("stored", {}).items():
+        out[f"fp{prec}_stored_w"] = brief(v, args.steps if prec == 32 else args.fp64_steps)
     for name, v in res.get("variants", {}).items():
-        out.setdefault("fp32_variants", {})[name] = {
-            "value": round(v["gpt_steps_per_s"], 4), "ms_per_step": round(v["ms_per_step"], 5),
-            "frac": round(v["achieved_Bps"] / HBM_PEAK, 4)}
+        out.setdefault("fp32_variants", {})[name] = brief(v, args.steps)
     traffic = os.path.join(ROOT, "profiles", f"traffic_m{n}.json")
     if os.path.exists(traffic):
         with open(traffic) as fh:

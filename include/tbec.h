@@ -152,8 +152,17 @@ int tbec_init_state(tbec_handle *h, const tbec_ic *ic);
 int tbec_set_norm(tbec_handle *h, double norm);
 
 /* Tile shape id (0..3, per precision) and x-segment length used by the stage kernel;
- * the autotuner (trapbec_amd/autotune.py) chooses them.  Drops captured graphs. */
+ * the autotuner (trapbec_amd/autotune.py) chooses them.  seg = 0 picks the longest
+ * segment (256 down to 8 planes) that still gives >= 1024 workgroups, which is also
+ * the default at creation.  Drops captured graphs. */
 int tbec_set_tuning(tbec_handle *h, int tile, int seg);
+/* How the low-storage RK3 carries w between stages (src/run.f90:183-191).
+ * stored = 1: the reference's scheme, w in its own array (10 HBM transfers of a complex
+ * value per point and step).  stored = 0 (default): w is rebuilt pointwise from the two
+ * psi buffers, w_{s-1} = (psi_s - psi_{s-1}) / B_{s-1} (8 transfers); the same RK3
+ * with one more rounding of psi per stage.  Real and imaginary time; batches always
+ * rebuild.  Drops captured graphs. */
+int tbec_set_w_storage(tbec_handle *h, int stored);
 /* Tile order: 1 (default) gives each of the 8 XCDs a contiguous block of tiles so that
  * neighbouring tiles share that XCD's L2; 0 keeps the dispatch order. */
 int tbec_set_xcd_remap(tbec_handle *h, int on);
@@ -180,6 +189,7 @@ int tbec_batch_download_psi(tbec_batch *b, int32_t i, void *psi_host);
  * own dtau = -dt_im); out receives n results. */
 int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double norm,
                           int check_every, tbec_solve_result *out);
+/* Tile and segment for every instance (seg = 0: automatic over the whole ensemble). */
 int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg);
 void *tbec_batch_stream(tbec_batch *b);
 

@@ -26,15 +26,19 @@ def announce(msg):
     print(f"\n[case] {msg}", flush=True)
 
 
+@pytest.mark.parametrize("stored", [False, True])
 @pytest.mark.parametrize("name", SMALL)
-def test_rk3_vs_reference_fixture(name):
+def test_rk3_vs_reference_fixture(name, stored):
+    """Both ways of carrying w (rebuilt from the psi pair, stored) against the
+    reference's own output."""
     f = np.load(os.path.join(GOLDEN, name + ".npz"))
     prec = int(f["prec"])
-    announce(f"{name} shape={f['psi0'].shape} prec={prec}")
+    announce(f"{name} shape={f['psi0'].shape} prec={prec} w_stored={stored}")
     box = tuple(float(b) for b in f["box"])
     with tbec.Condensate(f["psi0"].shape, box, tuple(float(h) for h in f["harm_w"]),
                          omega_z=float(f["omega_z"]), dt=complex(f["dt"]),
                          gamma=float(f["gamma"]), prec=prec) as c:
+        c.set_w_storage(stored)
         c.upload(f["psi0"])
         c.step(int(f["steps"]))
         got = c.download()
@@ -55,12 +59,14 @@ def test_euler_vs_reference_fixture():
     assert relmax(got, f["psi"]) < 1e-12
 
 
-def test_c1_full_size_vs_reference():
+@pytest.mark.parametrize("stored", [False, True])
+def test_c1_full_size_vs_reference(stored):
     """Config 1: 64^3 FP64, 1000 steps, against the 4-rank reference run and the oracle."""
     f = np.load(os.path.join(GOLDEN, "c1_f64_64.npz"))
-    announce("c1 64^3 fp64 1000 steps")
+    announce(f"c1 64^3 fp64 1000 steps w_stored={stored}")
     pr, psi0 = initial.config_c1(int(f["seed"]))
     with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.set_w_storage(stored)
         c.upload(psi0)
         c.step(int(f["steps"]))
         got = c.download()
@@ -80,9 +86,10 @@ def _rand_psi(shape, seed, scale=1.0):
 @pytest.mark.parametrize("shape,rot", [((37, 23, 50), 0.9), ((70, 19, 131), 0.0),
                                        ((8, 8, 8), -0.5)])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_ragged_grids_vs_oracle(shape, rot, prec):
+@pytest.mark.parametrize("stored", [False, True])
+def test_ragged_grids_vs_oracle(shape, rot, prec, stored):
     """Grids that are not multiples of any tile (masked stores, zero padding reads)."""
-    announce(f"ragged {shape} rot={rot} prec={prec}")
+    announce(f"ragged {shape} rot={rot} prec={prec} w_stored={stored}")
     box = (7.0, 5.0, 9.0)
     harm = (3.0, 2.0, 5.0)
     psi0 = _rand_psi(shape, 3, 0.3)
@@ -92,6 +99,7 @@ def test_ragged_grids_vs_oracle(shape, rot, prec):
     s.rk3(5)
     with tbec.Condensate(shape, box, harm, omega_z=rot, c0=0.7, gnl=1.3, dt=2e-4,
                          prec=prec) as c:
+        c.set_w_storage(stored)
         c.upload(psi0)
         r = c.rhs()
         c.step(5)
@@ -144,10 +152,11 @@ def test_diagnostics_vs_oracle():
     assert d["int_psi2"] == pytest.approx(info[3], rel=1e-12)
 
 
-def test_imag_rk3_renorm_vs_oracle():
+@pytest.mark.parametrize("stored", [False, True])
+def test_imag_rk3_renorm_vs_oracle(stored):
     """Renormalised, mu-shifted imaginary-time RK3 steps match the FP64 oracle."""
     shape, box, om, g = (24, 24, 24), (12.0, 12.0, 12.0), (1.0, 1.1, 0.9), 20.0
-    announce(f"imag rk3 {shape}")
+    announce(f"imag rk3 {shape} w_stored={stored}")
     pr = initial.Problem(shape, box, om, g, 0.3, -5e-3j)
     kp = pr.kernel_params()
     psi0 = initial.tf_profile(shape, box, om, g, norm=1.0)[0] * \
@@ -156,6 +165,7 @@ def test_imag_rk3_renorm_vs_oracle():
                                      c0=kp["c0"], gnl=kp["gnl"], dt=kp["dt"]), psi0)
     hist = s.imag_rk3(40, 1.0)
     with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.set_w_storage(stored)
         c.upload(psi0)
         c.step(40, tbec.IMAG_RK3)
         got = c.download()
@@ -237,31 +247,39 @@ def test_fp32_compensated_vs_fp64():
 
 
 def test_fp32_vs_fp64_512_1000_steps():
-    """The throughput workload itself: 512^3, 1000 steps; FP32 (compensated stencil sums)
-    stays within 1e-5 relative L2 of FP64, with norm and energy drifts reported."""
+    """The throughput workload itself: 512^3, 1000 steps.  Every FP32 variant (centred
+    and TwoSum compensated sums, reference order; w rebuilt or stored) stays within
+    1e-5 relative L2 of FP64 with the reference's scheme (stored w); norm and energy
+    errors against that run are printed.  FP64 with rebuilt w stays within 1e-12."""
     n, steps = 512, 1000
     announce(f"fp32 vs fp64 {n}^3 x {steps}")
     pr, psi0 = initial.config_m(n)
+    runs = [("f64 stored", 64, tbec.SUM_REFERENCE, True),
+            ("f64 rebuilt", 64, tbec.SUM_REFERENCE, False),
+            ("f32 centred rebuilt", 32, tbec.SUM_CENTRED, False),
+            ("f32 twosum rebuilt", 32, tbec.SUM_COMPENSATED, False),
+            ("f32 centred stored", 32, tbec.SUM_CENTRED, True),
+            ("f32 reference-order stored", 32, tbec.SUM_REFERENCE, True)]
     out, diag = {}, {}
-    for prec, sm in ((64, tbec.SUM_REFERENCE), (32, tbec.SUM_COMPENSATED),
-                     (32, tbec.SUM_REFERENCE)):
+    for name, prec, sm, stored in runs:
         with tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm) as c:
+            c.set_w_storage(stored)
             c.upload(psi0)
-            d0 = c.diagnose()
             c.step(steps)
-            d1 = c.diagnose()
-            out[(prec, sm)] = c.download()
-        diag[(prec, sm)] = (d0, d1)
+            diag[name] = c.diagnose()
+            out[name] = c.download()
     del psi0
-    ref = out[(64, 0)]
+    ref = out["f64 stored"]
     nref = float(np.linalg.norm(ref))
-
-    def l2(a):
-        return float(np.linalg.norm(a.astype(np.complex128) - ref) / nref)
-
-    e_comp, e_naive = l2(out[(32, 1)]), l2(out[(32, 0)])
-    for k, (d0, d1) in diag.items():
-        print(f"prec {k}: norm drift {abs(d1['norm'] - d0['norm']) / d0['norm']:.3e} "
-              f"energy drift {abs(d1['energy'] - d0['energy']) / abs(d0['energy']):.3e}")
-    print(f"512^3 x 1000 fp32 rel L2: compensated {e_comp:.3e} reference-order {e_naive:.3e}")
-    assert e_comp < 1e-5
+    dref = diag["f64 stored"]
+    errs = {}
+    for name, *_ in runs[1:]:
+        e = float(np.linalg.norm(out[name].astype(np.complex128) - ref) / nref)
+        errs[name] = e
+        d = diag[name]
+        print(f"512^3 x 1000 {name}: psi rel L2 {e:.3e}  norm rel "
+              f"{abs(d['norm'] - dref['norm']) / dref['norm']:.3e}  energy rel "
+              f"{abs(d['energy'] - dref['energy']) / abs(dref['energy']):.3e}", flush=True)
+    assert errs["f64 rebuilt"] < 1e-12
+    for name, e in errs.items():
+        assert e < 1e-5, name

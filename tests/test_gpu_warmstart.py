@@ -23,5 +23,8 @@ def test_upload_coarse_trilinear():
     ch = [map_coordinates(coarse[q].astype(np.float64), [X, Y, Z], order=1, mode="nearest")
           for q in range(3)]
     m = np.hypot(ch[1], ch[2])
-    want = np.maximum(ch[0], 0) * (ch[1] + 1j * ch[2]) / m
-    assert np.abs(got - want).max() < 2e-6
+    a = np.maximum(ch[0], 0)
+    want = a * (ch[1] + 1j * ch[2]) / m
+    # the device interpolates in FP32 (the network's output precision): an absolute
+    # error of a few FP32 ulps in each channel, amplified by a/m in the phase division
+    assert np.all(np.abs(got - want) < 2e-6 * (1.0 + a / m))

@@ -4,7 +4,7 @@ The stage kernel is compiled for four (TZ, TY, VZ) tile shapes per precision and
 the x-segment length SX at run time.  `tune` times every (tile, SX) candidate on the
 GPU for one grid shape with HIP events (libtbec's tbec_time_stages) and records the
 fastest in trapbec_amd/tuning.json; `lookup` returns the recorded choice, else a
-default.
+default tile with an automatic segment length (seg 0).
 """
 from __future__ import annotations
 
@@ -13,14 +13,14 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-DEFAULT = {32: (2, 128), 64: (2, 128)}
-SEGS = (32, 64, 128, 256)
+DEFAULT = {32: (2, 0), 64: (2, 0)}      # seg 0: libtbec picks it from the grid size
+SEGS = (8, 16, 32, 64, 128, 256)
 
 
-def _key(shape, prec, sum_mode=None):
+def _key(shape, prec, sum_mode=None, w_stored=False):
     if sum_mode is None:
-        sum_mode = 0 if prec == 64 else 1
-    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}:s{sum_mode}"
+        sum_mode = 0 if prec == 64 else 2
+    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}:s{sum_mode}" + (":ws" if w_stored else "")
 
 
 def _load():
@@ -30,23 +30,25 @@ def _load():
     return {}
 
 
-def lookup(shape, prec, sum_mode=None):
-    """(tile, seg, remap) recorded for this grid, precision and summation."""
-    t = _load().get(_key(shape, prec, sum_mode))
+def lookup(shape, prec, sum_mode=None, w_stored=False):
+    """(tile, seg, remap) recorded for this grid, precision, summation and w scheme."""
+    t = _load().get(_key(shape, prec, sum_mode, w_stored))
     if t:
         return int(t["tile"]), int(t["seg"]), int(t.get("remap", 1))
     return DEFAULT[prec] + (1,)
 
 
 def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
-         table=None):
-    """Time every (tile, seg) for this grid, precision and summation; return the entry."""
+         table=None, w_stored=False):
+    """Time every (tile, seg) for this grid, precision, summation and w scheme (stages
+    1-3 in turn); return the entry."""
     import numpy as np
     from trapbec_amd import tbec
     if sum_mode is None:
-        sum_mode = 0 if prec == 64 else 1
+        sum_mode = 0 if prec == 64 else 2
     c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
                         dt=1e-4, prec=prec, sum_mode=sum_mode)
+    c.set_w_storage(w_stored)
     if psi0 is None:
         psi0 = np.full(shape, 0.1 + 0.05j)
     c.upload(psi0)
@@ -69,10 +71,38 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
     if save:
         path = table or TABLE
         t = _load() if path == TABLE else (json.load(open(path)) if os.path.exists(path) else {})
-        t[_key(shape, prec, sum_mode)] = entry
+        t[_key(shape, prec, sum_mode, w_stored)] = entry
         with open(path, "w") as f:
             json.dump(t, f, indent=1, sort_keys=True)
     return entry
+
+
+def tune_batch(problems, prec=32, sum_mode=None, steps=20, verbose=False):
+    """Time every (tile, seg) for a batched ensemble: `steps` imaginary-time steps of all
+    instances (one captured graph, constant states, no convergence stop).  Returns
+    {"tile", "seg", "ms_per_step", "all"}; the caller applies it with Batch.set_tuning."""
+    import time
+    import numpy as np
+    from trapbec_amd import tbec
+    if steps & 1:
+        steps += 1
+    results = []
+    with tbec.Batch(problems, prec=prec, sum_mode=sum_mode) as b:
+        for i, pr in enumerate(problems):
+            b.upload(i, np.full(pr.shape, 0.1 + 0.05j))
+        for tile in range(4):
+            for seg in SEGS:
+                b.set_tuning(tile, seg)
+                b.imag_solve(steps, 0.0, 1.0, steps)          # capture, warm up
+                t0 = time.perf_counter()
+                b.imag_solve(steps, 0.0, 1.0, steps)
+                ms = (time.perf_counter() - t0) * 1e3 / steps
+                results.append((ms, tile, seg))
+                if verbose:
+                    print(f"batch tile {tile} seg {seg}: {ms:.3f} ms/step", flush=True)
+    ms, tile, seg = min(results)
+    return {"tile": tile, "seg": seg, "ms_per_step": ms,
+            "all": [[t, s, round(m, 4)] for m, t, s in sorted(results)]}
 
 
 if __name__ == "__main__":
@@ -81,7 +111,9 @@ if __name__ == "__main__":
     ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 512])
     ap.add_argument("--prec", type=int, default=32)
     ap.add_argument("--sum", type=int, default=None)
+    ap.add_argument("--w-stored", action="store_true", help="the stored-w RK3 scheme")
     ap.add_argument("--table", default=None, help="write here instead of tuning.json")
     a = ap.parse_args()
-    e = tune(tuple(a.shape), a.prec, a.sum, verbose=True, table=a.table)
-    print(json.dumps({"shape": a.shape, "prec": a.prec, "sum": a.sum, **e}))
+    e = tune(tuple(a.shape), a.prec, a.sum, verbose=True, table=a.table, w_stored=a.w_stored)
+    print(json.dumps({"shape": a.shape, "prec": a.prec, "sum": a.sum,
+                      "w_stored": a.w_stored, **e}), flush=True)

@@ -108,11 +108,26 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
 
 
 def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
+    from trapbec_amd import autotune
     probs = initial.config_c4(n_inst=n_inst)
     out = {"config": "c4", "instances": n_inst, "tol": tol, "max_steps": max_steps,
            "total_points": int(sum(np.prod(p.shape) for p, _ in probs))}
+    # tile shapes from the autotuner: one (tile, seg) for the ensemble, one per grid
+    # shape for the one-at-a-time solves
+    tb = autotune.tune_batch([p for p, _ in probs], prec=prec)
+    print(f"[c4] batch tuning: tile {tb['tile']} seg {tb['seg']} "
+          f"{tb['ms_per_step']:.3f} ms/step", flush=True)
+    single = {}
+    for shape in sorted({tuple(p.shape) for p, _ in probs}):
+        e = autotune.tune(shape, prec, save=False)
+        single[shape] = (e["tile"], e["seg"], e["remap"])
+        print(f"[c4] single tuning {shape}: tile {e['tile']} seg {e['seg']} "
+              f"remap {e['remap']}", flush=True)
+    out["tuning"] = {"batch": [tb["tile"], tb["seg"]],
+                     "single": {"x".join(map(str, k)): v for k, v in single.items()}}
     t0 = time.time()
     with tbec.Batch([p for p, _ in probs], prec=prec) as b:
+        b.set_tuning(tb["tile"], tb["seg"])
         for i, (_, psi0) in enumerate(probs):
             b.upload(i, psi0)
         t1 = time.time()
@@ -127,6 +142,9 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
     t_seq = 0.0
     for k, (pr, psi0) in enumerate(probs):
         with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            tile, seg, remap = single[tuple(pr.shape)]
+            c.set_tuning(tile, seg)
+            c.set_xcd_remap(remap)
             c.upload(psi0)
             c.sync()
             t1 = time.time()

@@ -8,6 +8,17 @@
 // (src/run.f90:183-191), writing psi_out and w.  Variants: the reference imaginary
 // Euler update psi + (dt/2)*rhs (src/run.f90:159) and a rhs-only pass (test hook).
 //
+// Two ways to carry w between stages.  Stored (the reference's scheme): stages 1 and 2
+// write w, stages 2 and 3 read it: 10 HBM transfers of a complex value per point and
+// step.  Rebuilt: w is not stored.  The stage input psi_s and the previous stage's input
+// psi_{s-1} (still held in the other ping-pong buffer) give it back pointwise:
+// w_{s-1} = (psi_s - psi_{s-1}) / B_{s-1}, since psi_s = psi_{s-1} + B_{s-1} w_{s-1}.
+// Stage 2 reads psi_1 (stencil) and psi_0 and writes psi_2 over psi_0; stage 3 reads
+// psi_2 and psi_1 and writes psi_3 over psi_1 (each point is read and rewritten by the
+// same thread only).  That is 8 HBM transfers of a complex value per point and step
+// instead of the 10 of a stored w.  In imaginary time stage 1 reads scale*psi_0; stage
+// 2 forms the same product (same rounding) from psi_0 before the difference.
+//
 // Layout (per condensate): psi, w are padded arrays, z fastest.  Interior (0,0,0) sits
 // at `base`; strides sy (y) and sx (x) in complex elements.  The halo (3 cells, more at
 // row ends) is zero and never written, which is the reference's zero boundary
@@ -63,9 +74,14 @@ struct StageArgs {
   double c0, gnl, om;
   double cre, cim;             // complex update coefficient
   double A, B;                 // stage coefficients
+  double wrec;                 // 0: read w from w_buf.  Else (stages 2, 3, rebuilt w):
+                               // w = (psi - pscale*w_buf) * wrec, where w_buf holds the
+                               // previous stage's input
+  int store_w;                 // write w (stored scheme, stages 1 and 2)
   int stage;                   // StageKind
   int remap;                   // 1: spread tiles over XCDs in contiguous chunks
   const double *scale;         // imaginary time: psi is read as scale*psi (or null)
+  const double *prev_scale;    // imaginary time, stage 2: scale of w_buf (or null)
   const double *c0dev;         // imaginary time: c0 from device memory (or null)
   const int *skip;             // imaginary time: converged flag; skip the pass when set
   double *partials;            // reductions: 4 doubles per workgroup
@@ -264,13 +280,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const long long sx = a.sx, sy = a.sy;
   const long long rb = a.base - 3 * sy - 4;
   const C *__restrict__ pin = reinterpret_cast<const C *>(a.psi_in) + rb;
-  C *__restrict__ pout = reinterpret_cast<C *>(a.psi_out) + rb;
-  C *__restrict__ pw = reinterpret_cast<C *>(a.w) + rb;
+  // in stages 2, 3 psi_out and w are the same buffer (read, then rewritten, pointwise)
+  C *pout = reinterpret_cast<C *>(a.psi_out) + rb;
+  C *pw = reinterpret_cast<C *>(a.w) + rb;
 
-  T sc = T(1);
+  T sc = T(1), psc = T(1);
   T c0 = T(a.c0);
   if constexpr (IMAG) {
     if (a.scale) sc = T(*a.scale);
+    if (a.prev_scale) psc = T(*a.prev_scale);
     if (a.c0dev) c0 = T(*a.c0dev);
   }
 
@@ -337,6 +355,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const T wlx = T(a.wl[0]), wly = T(a.wl[1]), wlz = T(a.wl[2]);
   const T gnl = T(a.gnl), om = T(a.om);
   const T cre = T(a.cre), cim = T(a.cim), A = T(a.A), B = T(a.B);
+  const T wrec = T(a.wrec);
+  const bool rebuild = a.wrec != 0.0;
+  const bool store_w = a.store_w != 0;
   const int stage = a.stage;
   const T yc = T(a.y0 + a.hy * y);
   T zc2[VZ];
@@ -432,7 +453,12 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         wo[v] = {ur, ui};
         out[v] = {f.re + B * ur, f.im + B * ui};
       } else {
-        wo[v] = {A * wv[v].re + ur, A * wv[v].im + ui};
+        C wp = wv[v];
+        if (rebuild) {
+          if constexpr (IMAG) { wp.re *= psc; wp.im *= psc; }
+          wp = {(f.re - wp.re) * wrec, (f.im - wp.im) * wrec};
+        }
+        wo[v] = {A * wp.re + ur, A * wp.im + ui};
         out[v] = {f.re + B * wo[v].re, f.im + B * wo[v].im};
       }
       if constexpr (IMAG) {
@@ -458,7 +484,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
           BufIO<T, VZ>::st(rs_w(x), ownb, wo);
         } else {
           BufIO<T, VZ>::st(rs_out(x), ownb, out);
-          if (stage == ST_RK1 || stage == ST_RK2) BufIO<T, VZ>::st(rs_w(x), ownb, wo);
+          if (store_w) BufIO<T, VZ>::st(rs_w(x), ownb, wo);
         }
       } else {
 #pragma unroll
@@ -468,7 +494,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
             pwx[own + v] = wo[v];
           } else {
             po[own + v] = out[v];
-            if (stage == ST_RK1 || stage == ST_RK2) pwx[own + v] = wo[v];
+            if (store_w) pwx[own + v] = wo[v];
           }
         }
       }
@@ -557,6 +583,10 @@ gpe_stage_batch(const BatchLaunch L) {
     else { a.cre = -0.5 * bi.dt_im; a.cim = 0.5 * bi.dt_re; }
     const double *st = L.state + (long long)i * L.state_stride;
     a.scale = (IMAG && L.stage == ST_RK1) ? st + 0 : nullptr;   // ST_SCALE
+    a.prev_scale = (IMAG && L.stage == ST_RK2) ? st + 0 : nullptr;
+    a.w = a.psi_out;                     // w rebuilt from the psi pair (stages 2, 3)
+    a.wrec = L.stage == ST_RK2 ? 3.0 : (L.stage == ST_RK3 ? 1.0 : 0.0);
+    a.store_w = 0;
     a.c0dev = IMAG ? st + 1 : nullptr;                            // ST_C0EFF
     a.skip = nullptr;
     double red[4] = {0.0, 0.0, 0.0, 0.0};

@@ -60,6 +60,19 @@ const TileCfg &tile_cfg(int prec, int id) {
   return prec == TBEC_F32 ? kTilesF32[id] : kTilesF64[id];
 }
 
+long long tiles_per_plane(const tbec_params &p, int tile) {
+  const TileCfg &t = tile_cfg(p.prec, tile);
+  return (long long)((p.nz + t.tz - 1) / t.tz) * ((p.ny + t.ty - 1) / t.ty);
+}
+
+// Segment length when none is given: the longest of 256, 128, ..., 8 planes that still
+// gives `want` workgroups (a few per CU on 256 CUs); 8 for grids too small for that.
+int auto_seg(long long tiles, int nx, long long want) {
+  for (int seg = 256; seg > 8; seg >>= 1)
+    if (tiles * ((nx + seg - 1) / seg) >= want) return seg;
+  return 8;
+}
+
 hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool rot,
                         bool imag, hipStream_t s) {
   if (prec == TBEC_F32) {
@@ -240,9 +253,12 @@ int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, S
   a.wl[2] = p.harm_wz / (p.Lz * p.Lz);
   a.c0 = p.c0; a.gnl = p.gnl; a.om = p.omega_z;
   a.cre = a.cim = a.A = a.B = 0.0;
+  a.wrec = 0.0;
+  a.store_w = 0;
   a.stage = 0;
   a.remap = 1;
-  a.scale = nullptr; a.c0dev = nullptr; a.skip = nullptr; a.partials = nullptr;
+  a.scale = nullptr; a.prev_scale = nullptr; a.c0dev = nullptr; a.skip = nullptr;
+  a.partials = nullptr;
   return a.nblocks;
 }
 
@@ -308,8 +324,17 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     a.c0dev = h->dstate + ST_C0EFF;
     a.skip = h->dflag;
     if (kind == ST_RK1) a.scale = h->dstate + ST_SCALE;
+    if (kind == ST_RK2) a.prev_scale = h->dstate + ST_SCALE;
   } else if (reduce) {
     a.scale = h->dstate + ST_SCALE;
+  }
+  if (h->w_stored) {
+    a.store_w = kind == ST_RK1 || kind == ST_RK2;
+  } else if (kind == ST_RK2 || kind == ST_RK3) {
+    // w of the previous stage comes back from that stage's input, which is in the
+    // buffer this stage overwrites (1/B_1 = 3, 1/B_2 = 1)
+    a.w = a.psi_out;
+    a.wrec = kind == ST_RK2 ? 3.0 : 1.0;
   }
   hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, handle_rot(h), want_red,
                               h->stream);
@@ -364,7 +389,7 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   tbec_handle *h = new tbec_handle();
   h->p = *p;
   h->tile = p->tile < 0 ? 0 : p->tile;
-  h->seg = 64;
+  h->seg = auto_seg(tiles_per_plane(*p, h->tile), p->nx, kAutoSegBlocks);
   Layout L;
   compute_layout(*p, L);
   h->csize = L.csize;
@@ -560,11 +585,20 @@ int tbec_diagnose(tbec_handle *h, tbec_diag *out) {
 int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
   if (!h) return fail(TBEC_ERR_ARG, "null handle");
   if (tile < 0 || tile >= kNumTiles) return fail(TBEC_ERR_ARG, "tile id %d out of range", tile);
-  if (seg < 1 || seg > 4096) return fail(TBEC_ERR_ARG, "segment length %d out of range", seg);
+  if (seg < 0 || seg > 4096) return fail(TBEC_ERR_ARG, "segment length %d out of range", seg);
   HIPCHK(hipStreamSynchronize(h->stream));
   tbec_drop_graphs(h);
   h->tile = tile;
-  h->seg = seg;
+  h->seg = seg > 0 ? seg : auto_seg(tiles_per_plane(h->p, tile), h->p.nx, kAutoSegBlocks);
+  return 0;
+}
+
+int tbec_set_w_storage(tbec_handle *h, int stored) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (stored != 0 && stored != 1) return fail(TBEC_ERR_ARG, "stored must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  tbec_drop_graphs(h);
+  h->w_stored = stored;
   return 0;
 }
 

@@ -32,7 +32,7 @@ struct tbec_batch {
   bool rot = false;
   std::vector<tbec_params> p;
   std::vector<Layout> L;
-  std::vector<void *> bufs;          // 3 per instance: psi0, psi1, w
+  std::vector<void *> bufs;          // 2 per instance: the psi ping-pong pair
   std::vector<int> item_begin;       // n + 1
   std::vector<int> cur;              // per-instance current buffer after a solve
   int parity = 0;
@@ -123,6 +123,18 @@ int grid_blocks(long long n) {
   return (int)std::max<long long>(1, std::min<long long>(b, 8192));
 }
 
+// Automatic segment length for the whole ensemble: the items of all instances together
+// should number a few per CU; the largest instance's x extent bounds the search.
+int batch_auto_seg(const tbec_batch *b, int tile) {
+  for (int seg = 256; seg > 8; seg >>= 1) {
+    long long items = 0;
+    for (int i = 0; i < b->n; ++i)
+      items += tiles_per_plane(b->p[i], tile) * ((b->p[i].nx + seg - 1) / seg);
+    if (items >= kAutoSegBlocks) return seg;
+  }
+  return 8;
+}
+
 // Build the device instance table for the current tile / segment choice.
 int build_table(tbec_batch *b) {
   std::vector<BatchInst> inst(b->n);
@@ -130,9 +142,9 @@ int build_table(tbec_batch *b) {
   for (int i = 0; i < b->n; ++i) {
     BatchInst &bi = inst[i];
     const int nt = fill_static_args(b->p[i], b->L[i], b->tile, b->seg, bi.a);
-    bi.a.w = b->bufs[3 * i + 2];
-    bi.psi[0] = b->bufs[3 * i];
-    bi.psi[1] = b->bufs[3 * i + 1];
+    bi.a.w = nullptr;               // w is rebuilt from the psi pair (stages 2, 3)
+    bi.psi[0] = b->bufs[2 * i];
+    bi.psi[1] = b->bufs[2 * i + 1];
     bi.dt_re = b->p[i].dt_re;
     bi.dt_im = b->p[i].dt_im;
     bi.item_begin = b->item_begin[i];
@@ -289,7 +301,7 @@ int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out) {
     return cleanup(set_error(TBEC_ERR_HIP, "stream creation failed"));
   for (int i = 0; i < n; ++i) {
     const size_t bytes = (size_t)b->L[i].elems * b->L[i].csize;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
       void *ptr = nullptr;
       if (hipMalloc(&ptr, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -310,6 +322,7 @@ int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out) {
     (void)hipGetLastError();
     return cleanup(set_error(TBEC_ERR_OOM, "batch state allocation failed"));
   }
+  b->seg = batch_auto_seg(b, b->tile);
   int rc = build_table(b);
   if (!rc) rc = reset_states(b, 1.0, 0.0);
   if (rc) return cleanup(rc);
@@ -338,12 +351,12 @@ int tbec_batch_destroy(tbec_batch *b) {
 
 int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg) {
   if (!b) return set_error(TBEC_ERR_ARG, "null batch");
-  if (tile < 0 || tile >= kNumTiles || seg < 1 || seg > 4096)
+  if (tile < 0 || tile >= kNumTiles || seg < 0 || seg > 4096)
     return set_error(TBEC_ERR_ARG, "tile or segment out of range");
   BCHK(hipStreamSynchronize(b->stream));
   drop_graphs(b);
   b->tile = tile;
-  b->seg = seg;
+  b->seg = seg > 0 ? seg : batch_auto_seg(b, tile);
   return build_table(b);
 }
 
@@ -355,7 +368,7 @@ int tbec_batch_upload_psi(tbec_batch *b, int32_t i, const void *psi_host) {
   const Layout &L = b->L[i];
   const long long n = (long long)q.nx * q.ny * q.nz;
   BCHK(hipMemcpyAsync(b->stage, psi_host, n * L.csize, hipMemcpyHostToDevice, b->stream));
-  void *dst = b->bufs[3 * i + b->parity];
+  void *dst = b->bufs[2 * i + b->parity];
   if (b->prec == TBEC_F32)
     hipLaunchKernelGGL(k_batch_unpack<float>, dim3(grid_blocks(n)), dim3(256), 0, b->stream,
                        (const cplx<float> *)b->stage, (cplx<float> *)dst, q.nx, q.ny, q.nz,
@@ -375,7 +388,7 @@ int tbec_batch_download_psi(tbec_batch *b, int32_t i, void *psi_host) {
   const tbec_params &q = b->p[i];
   const Layout &L = b->L[i];
   const long long n = (long long)q.nx * q.ny * q.nz;
-  const void *src = b->bufs[3 * i + b->cur[i]];
+  const void *src = b->bufs[2 * i + b->cur[i]];
   const double *scale = b->d_state + (size_t)i * ST_COUNT + ST_SCALE;
   if (b->prec == TBEC_F32)
     hipLaunchKernelGGL(k_batch_pack<float>, dim3(grid_blocks(n)), dim3(256), 0, b->stream,

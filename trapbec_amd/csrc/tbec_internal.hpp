@@ -43,6 +43,9 @@ bool rot_enabled(const tbec_params &p);
 long long round_up(long long v, long long m);
 const TileCfg &tile_cfg(int prec, int id);
 extern const int kNumTiles;
+long long tiles_per_plane(const tbec_params &p, int tile);
+int auto_seg(long long tiles, int nx, long long want);
+constexpr long long kAutoSegBlocks = 1024;
 
 // A solve has converged after this many consecutive steps with a relative energy
 // change below the tolerance (one step can dip below it by rounding noise in FP32).
@@ -58,6 +61,7 @@ struct tbec_handle {
   tbec_params p{};
   int tile = 0, seg = 64;
   int remap = 1;                        // XCD-aware tile order
+  int w_stored = 0;                     // 1: RK3 stores w (10 transfers), 0: rebuilt (8)
   int csize = 8;
   double hx = 0, hy = 0, hz = 0;
   double dt_re = 0, dt_im = 0;          // dt*(1 - i*gamma)

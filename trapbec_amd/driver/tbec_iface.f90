@@ -9,7 +9,8 @@ module tbec_iface
   implicit none
 
   integer(c_int), parameter :: TBEC_F32 = 32, TBEC_F64 = 64
-  integer(c_int), parameter :: TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1
+  integer(c_int), parameter :: TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1, &
+                                TBEC_SUM_CENTRED = 2
   integer(c_int), parameter :: TBEC_REAL_RK3 = 0, TBEC_IMAG_RK3 = 1, TBEC_IMAG_EULER_REF = 2
 
   type, bind(C) :: tbec_params

@@ -69,7 +69,7 @@ program tbec_run
   if (shots < 1) call emergency_stop('shots must be positive')
   if (sum_mode < 0) then
     sum_mode = TBEC_SUM_REFERENCE
-    if (prec == 32) sum_mode = TBEC_SUM_COMPENSATED
+    if (prec == 32) sum_mode = TBEC_SUM_CENTRED
   end if
 
   write(*, '(a)') ' TrapBEC (MI355X) driver initialised'

@@ -69,7 +69,8 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
-           "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state"]
+           "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
+           "tbec_set_w_storage"]
 
 _lib = None
 
@@ -136,7 +137,7 @@ class Condensate:
                  gamma=0.0, prec=64, sum_mode=None, tile=-1):
         L = load()
         if sum_mode is None:
-            sum_mode = SUM_REFERENCE if prec == 64 else SUM_COMPENSATED
+            sum_mode = SUM_REFERENCE if prec == 64 else SUM_CENTRED
         box = (box, box, box) if np.isscalar(box) else tuple(box)
         dt = complex(dt)
         self.shape, self.box, self.prec = tuple(shape), box, prec
@@ -241,6 +242,11 @@ class Condensate:
     def set_xcd_remap(self, on: bool):
         _check(load().tbec_set_xcd_remap(self._h, int(bool(on))))
 
+    def set_w_storage(self, stored: bool):
+        """True: RK3 keeps w in its own array (the reference's scheme, 10 transfers per
+        point-step); False (default): w is rebuilt from the two psi buffers (8)."""
+        _check(load().tbec_set_w_storage(self._h, int(bool(stored))))
+
     def sync(self):
         _check(load().tbec_sync(self._h))
 
@@ -262,7 +268,7 @@ class Batch:
     def __init__(self, problems, prec=32, sum_mode=None, tile=-1):
         L = load()
         if sum_mode is None:
-            sum_mode = SUM_REFERENCE if prec == 64 else SUM_COMPENSATED
+            sum_mode = SUM_REFERENCE if prec == 64 else SUM_CENTRED
         self.n = len(problems)
         self.prec = prec
         self.cdt = np.complex128 if prec == 64 else np.complex64
