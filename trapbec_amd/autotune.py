@@ -55,7 +55,7 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
     results = []
     for remap in (1, 0):
         c.set_xcd_remap(remap)
-        for tile in range(4):
+        for tile in range(tbec.NUM_TILES):
             for seg in SEGS:
                 c.set_tuning(tile, seg)
                 c.time_stages(3)
@@ -90,7 +90,7 @@ def tune_batch(problems, prec=32, sum_mode=None, steps=20, verbose=False):
     with tbec.Batch(problems, prec=prec, sum_mode=sum_mode) as b:
         for i, pr in enumerate(problems):
             b.upload(i, np.full(pr.shape, 0.1 + 0.05j))
-        for tile in range(4):
+        for tile in range(tbec.NUM_TILES):
             for seg in SEGS:
                 b.set_tuning(tile, seg)
                 b.imag_solve(steps, 0.0, 1.0, steps)          # capture, warm up
