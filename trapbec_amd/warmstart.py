@@ -69,7 +69,9 @@ def coarse_field(psi: np.ndarray, cn: int = COARSE) -> np.ndarray:
 
 def generate(n_samples, seed=11, n=64, dtau=5e-3, tol=1e-8, max_steps=8000, chunk=250,
              log=print):
-    """Ground states for Sobol parameters via the batched FP32 imaginary-time solver."""
+    """Ground states for Sobol parameters via the batched FP32 imaginary-time solver.
+    Every instance starts from the TF profile with the same seeded phase noise, so the
+    state reached (vortex positions included) is a function of the parameters alone."""
     params = sobol_params(n_samples, seed)
     X = features(params)
     Y = np.zeros((n_samples, 3 * COARSE ** 3), np.float32)
@@ -83,7 +85,7 @@ def generate(n_samples, seed=11, n=64, dtau=5e-3, tol=1e-8, max_steps=8000, chun
         t0 = time.time()
         with tbec.Batch(probs, prec=32) as b:
             for k, i in enumerate(idx):
-                b.upload(k, tf_start(probs[k], 1.0, seed=i))
+                b.upload(k, tf_start(probs[k], 1.0, seed=0))
             res = b.imag_solve(max_steps, tol, 1.0, 100)
             for k, i in enumerate(idx):
                 Y[i] = coarse_field(b.download(k) * math.sqrt(norms[k])).ravel()
@@ -132,7 +134,8 @@ def main():
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     t0 = time.time()
-    params, X, Y, stats = generate(a.samples, n=a.train_n)
+    params, X, Y, stats = generate(a.samples, n=a.train_n,
+                                   log=lambda m: print(m, flush=True))
     t_gen = time.time() - t0
     scale = np.ones(Y.shape[1], np.float32)
     amp = Y[:, :COARSE ** 3].max()
@@ -143,14 +146,21 @@ def main():
         t1 = time.time()
         hist = m.train(a.epochs, a.batch, a.lr, seed=2)
         t_train = time.time() - t1
+        print(f"[warmstart] trained {a.epochs} epochs in {t_train:.1f} s, loss "
+              f"{hist[0]:.3e} -> {hist[-1]:.3e}", flush=True)
         val_pred = m.forward(X[n_tr:]) / scale
         val_mse = float(np.mean((val_pred - Y[n_tr:]) ** 2))
         base_mse = float(np.mean((Y[:n_tr].mean(axis=0) - Y[n_tr:]) ** 2))
         test = np.array([1.0, 0.8, 3000.0, 0.5, 0.0])
         coarse = (m.forward(features(test[None]))[0] / scale).reshape(3, COARSE, COARSE, COARSE)
+    # the network predicts the state at atom number N; the refinement runs at norm 1
+    coarse[0] /= math.sqrt(math.exp(test[4]))
     pr = fine_problem(test, a.fine, a.fine_dtau)
-    r_tf = refine(pr, tf_start(pr, 1.0, seed=99), a.tol, a.max_steps)
+    print(f"[warmstart] refine {a.fine}^3 from the TF start", flush=True)
+    r_tf = refine(pr, tf_start(pr, 1.0, seed=0), a.tol, a.max_steps)
+    print(f"[warmstart] TF start: {r_tf['steps']} steps {r_tf['wall_s']:.1f} s", flush=True)
     r_ml = refine(pr, None, a.tol, a.max_steps, coarse=coarse)
+    print(f"[warmstart] MLP start: {r_ml['steps']} steps {r_ml['wall_s']:.1f} s", flush=True)
     out = {"samples": a.samples, "gen_s": t_gen, "train_s": t_train,
            "train_loss_first_last": [float(hist[0]), float(hist[-1])],
            "val_mse": val_mse, "val_mse_mean_predictor": base_mse,
