@@ -171,7 +171,9 @@ int tbec_sync(tbec_handle *h);
 /* The handle's HIP stream (hipStream_t), for event timing by the caller. */
 void *tbec_stream(tbec_handle *h);
 /* Average duration (ms) of one stage kernel measured with HIP events over n stages
- * launched back to back on the handle's stream (real-time RK3 stages). */
+ * launched back to back on the handle's stream: real-time RK3 stages, or, when the
+ * handle's dt is (0, -dtau), the imaginary-time stages with their reductions and
+ * finalisers (this advances psi).  For the autotuner. */
 int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage);
 
 /* ---- batched ensembles: many independent condensates advanced together ---------

@@ -39,15 +39,20 @@ def lookup(shape, prec, sum_mode=None, w_stored=False):
 
 
 def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
-         table=None, w_stored=False):
+         table=None, w_stored=False, problem=None):
     """Time every (tile, seg) for this grid, precision, summation and w scheme (stages
-    1-3 in turn); return the entry."""
+    1-3 in turn); return the entry.  With `problem` (an initial.Problem) the stages of
+    that problem are timed (rotation, imaginary time) and nothing is saved."""
     import numpy as np
     from trapbec_amd import tbec
     if sum_mode is None:
         sum_mode = 0 if prec == 64 else 2
-    c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
-                        dt=1e-4, prec=prec, sum_mode=sum_mode)
+    if problem is not None:
+        shape, save = tuple(problem.shape), False
+        c = tbec.Condensate.from_problem(problem, prec=prec, sum_mode=sum_mode)
+    else:
+        c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
+                            dt=1e-4, prec=prec, sum_mode=sum_mode)
     c.set_w_storage(w_stored)
     if psi0 is None:
         psi0 = np.full(shape, 0.1 + 0.05j)

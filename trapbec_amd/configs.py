@@ -82,10 +82,16 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
     pr, psi0 = initial.config_c3(n=n)
     out = {"config": "c3", "grid": n, "tol": tol}
     res = {}
+    from trapbec_amd import autotune
     for prec in (64, 32):
+        e = autotune.tune(None, prec, problem=pr)
         with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.set_tuning(e["tile"], e["seg"])
+            c.set_xcd_remap(e["remap"])
             c.upload(psi0)
-            print(f"[c3] fp{prec} imaginary-time solve {pr.shape}", flush=True)
+            print(f"[c3] fp{prec} imaginary-time solve {pr.shape}, tile {e['tile']} seg "
+                  f"{e['seg']} remap {e['remap']}: {e['us_per_stage']:.1f} us/stage",
+                  flush=True)
             t0 = time.time()
             r = c.imag_solve(max_steps, tol, 1.0, check_every)
             c.sync()
@@ -119,7 +125,8 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
           f"{tb['ms_per_step']:.3f} ms/step", flush=True)
     single = {}
     for shape in sorted({tuple(p.shape) for p, _ in probs}):
-        e = autotune.tune(shape, prec, save=False)
+        first = next(p for p, _ in probs if tuple(p.shape) == shape)
+        e = autotune.tune(None, prec, problem=first)
         single[shape] = (e["tile"], e["seg"], e["remap"])
         print(f"[c4] single tuning {shape}: tile {e['tile']} seg {e['seg']} "
               f"remap {e['remap']}", flush=True)

@@ -388,7 +388,7 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   HIPCHK(hipGetDevice(&dev));
   tbec_handle *h = new tbec_handle();
   h->p = *p;
-  h->tile = p->tile < 0 ? 0 : p->tile;
+  h->tile = p->tile < 0 ? 2 : p->tile;
   h->seg = auto_seg(tiles_per_plane(*p, h->tile), p->nx, kAutoSegBlocks);
   Layout L;
   compute_layout(*p, L);

@@ -288,7 +288,7 @@ int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out) {
   b->n = n;
   b->prec = params[0].prec;
   b->sum = params[0].sum_mode;
-  b->tile = params[0].tile < 0 ? 0 : params[0].tile;
+  b->tile = params[0].tile < 0 ? 2 : params[0].tile;
   b->p.assign(params, params + n);
   b->L.resize(n);
   b->cur.assign(n, 0);

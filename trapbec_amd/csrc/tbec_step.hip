@@ -210,12 +210,21 @@ int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage) {
   if (!h || !ms_per_stage || nstages < 1) return set_error(TBEC_ERR_ARG, "bad argument");
   int rc = tbec_materialize_scale(h);
   if (rc) return rc;
+  // a handle with dt = (0, -dtau) times the imaginary-time stages (reductions and
+  // finalisers included) from a fresh solver state with no convergence stop
+  const bool imag = h->p.dt_re == 0.0 && h->p.dt_im < 0.0;
+  if (imag) {
+    h->tol = 0.0;
+    rc = tbec_reset_state(h);
+    if (rc) return rc;
+    h->scale_pending = true;
+  }
   hipEvent_t e0, e1;
   HIPCHK2(hipEventCreate(&e0));
   HIPCHK2(hipEventCreate(&e1));
   HIPCHK2(hipEventRecord(e0, h->stream));
   for (int i = 0; i < nstages; ++i) {
-    rc = enqueue_stage(h, ST_RK1 + (i % 3), false, false);
+    rc = enqueue_stage(h, ST_RK1 + (i % 3), imag, false);
     if (rc) return rc;
   }
   HIPCHK2(hipEventRecord(e1, h->stream));
@@ -225,7 +234,7 @@ int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   *ms_per_stage = ms / nstages;
-  return 0;
+  return imag ? tbec_materialize_scale(h) : 0;
 }
 
 }  // extern "C"

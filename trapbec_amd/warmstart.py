@@ -21,7 +21,7 @@ import numpy as np
 
 os.environ.setdefault("TBEC_PROGRESS", "20")
 
-from trapbec_amd import initial, tbec  # noqa: E402
+from trapbec_amd import autotune, initial, tbec  # noqa: E402
 
 BOX = 24.0
 RANGES = np.array([[0.8, 1.2],      # w_perp
@@ -102,9 +102,13 @@ def fine_problem(p, n, dtau):
     return problem(np.array([wp, wz, g * math.exp(logN), Om, 0.0]), n, dtau)
 
 
-def refine(pr, start, tol, max_steps, check_every=200, coarse=None, prec=32):
-    """FP32 imaginary time to tol from a psi start (or a coarse field); returns stats."""
+def refine(pr, start, tol, max_steps, check_every=200, coarse=None, prec=32, tuning=None):
+    """FP32 imaginary time to tol from a psi start (or a coarse field); returns stats.
+    tuning: (tile, seg, remap) from autotune.tune(problem=pr)."""
     with tbec.Condensate.from_problem(pr, prec=prec) as c:
+        if tuning is not None:
+            c.set_tuning(tuning[0], tuning[1])
+            c.set_xcd_remap(tuning[2])
         if coarse is not None:
             tbec.upload_coarse(c, coarse)
         else:
@@ -156,10 +160,13 @@ def main():
     # the network predicts the state at atom number N; the refinement runs at norm 1
     coarse[0] /= math.sqrt(math.exp(test[4]))
     pr = fine_problem(test, a.fine, a.fine_dtau)
+    e = autotune.tune(None, 32, problem=pr)
+    tuning = (e["tile"], e["seg"], e["remap"])
+    print(f"[warmstart] tuning {tuning}: {e['us_per_stage']:.1f} us/stage", flush=True)
     print(f"[warmstart] refine {a.fine}^3 from the TF start", flush=True)
-    r_tf = refine(pr, tf_start(pr, 1.0, seed=0), a.tol, a.max_steps)
+    r_tf = refine(pr, tf_start(pr, 1.0, seed=0), a.tol, a.max_steps, tuning=tuning)
     print(f"[warmstart] TF start: {r_tf['steps']} steps {r_tf['wall_s']:.1f} s", flush=True)
-    r_ml = refine(pr, None, a.tol, a.max_steps, coarse=coarse)
+    r_ml = refine(pr, None, a.tol, a.max_steps, coarse=coarse, tuning=tuning)
     print(f"[warmstart] MLP start: {r_ml['steps']} steps {r_ml['wall_s']:.1f} s", flush=True)
     out = {"samples": a.samples, "gen_s": t_gen, "train_s": t_train,
            "train_loss_first_last": [float(hist[0]), float(hist[-1])],
