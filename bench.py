@@ -273,6 +273,7 @@ def main():
     if os.path.exists(traffic):
         with open(traffic) as fh:
             out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch_fp32")
+            out["roofline"]["traffic_unit"] = "bytes per launch (FP32 headline kernel)"
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         try:
