@@ -58,7 +58,7 @@ def run_c2(n=256, steps=2000):
             wall = time.time() - t0
             d1 = c.diagnose()
             finals[prec] = c.download().astype(np.complex128)
-            cores = _cores(c, 1e-3 * float(np.max(np.abs(psi0)) ** 2))
+            cores = _cores(c, 1e-5 * float(np.max(np.abs(psi0)) ** 2))
         diags[prec] = {"norm0": d0["norm"], "norm1": d1["norm"], "E0": d0["energy"],
                        "E1": d1["energy"], "wall_s": wall, "n_cores_z": int(len(cores))}
         finals[(prec, "cores")] = cores
@@ -101,7 +101,7 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
             r["final_residual"] = d["residual"]
             psi = c.download()
             dens = float(np.max(np.abs(psi)) ** 2)
-            cores = c.vortex_cores(1e-3 * dens, 0)
+            cores = c.vortex_cores(1e-5 * dens, 0)
             mid = n // 2
             r["vortex_count_midplane"] = int(np.sum(cores[:, 2] == mid)) if len(cores) else 0
             r["vortex_charge_sum_midplane"] = int(np.sum(cores[cores[:, 2] == mid][:, 3])) \
