@@ -151,7 +151,7 @@ int tbec_init_state(tbec_handle *h, const tbec_ic *ic);
 /* Set the norm target used by TBEC_IMAG_RK3 steps (default 1). */
 int tbec_set_norm(tbec_handle *h, double norm);
 
-/* Tile shape id (0..3, per precision) and x-segment length used by the stage kernel;
+/* Tile shape id (0..4, per precision) and x-segment length used by the stage kernel;
  * the autotuner (trapbec_amd/autotune.py) chooses them.  seg = 0 picks the longest
  * segment (256 down to 8 planes) that still gives >= 1024 workgroups, which is also
  * the default at creation.  Drops captured graphs. */

@@ -112,7 +112,7 @@ def test_ragged_grids_vs_oracle(shape, rot, prec, stored):
         assert relmax(got, s.interior) < 1e-5
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_every_tile_shape(tile, prec):
     shape = (40, 72, 136)

@@ -1,6 +1,6 @@
 """Tile-shape autotuner for the fused stage kernel.
 
-The stage kernel is compiled for four (TZ, TY, VZ) tile shapes per precision and takes
+The stage kernel is compiled for five (TZ, TY, VZ) tile shapes per precision and takes
 the x-segment length SX at run time.  `tune` times every (tile, SX) candidate on the
 GPU for one grid shape with HIP events (libtbec's tbec_time_stages) and records the
 fastest in trapbec_amd/tuning.json; `lookup` returns the recorded choice, else a

@@ -161,34 +161,50 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, long
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, n, kRsrcWord3);
 }
 
+typedef unsigned int tbec_u32x2 __attribute__((ext_vector_type(2)));
+
 template <typename T, int VZ> struct BufIO {
-  static constexpr int NB = VZ * 2 * (int)sizeof(T) / 16;   // 16-byte chunks
+  // VZ complex values per thread: 16-byte chunks, or one 8-byte access for a single
+  // complex<float>
+  static constexpr int BYTES = VZ * 2 * (int)sizeof(T);
+  static constexpr int NB = BYTES / 16;
+  static_assert(BYTES == 8 || BYTES % 16 == 0, "unsupported vector width");
   static __device__ __forceinline__ void ld(__amdgpu_buffer_rsrc_t r, uint32_t off,
                                             cplx<T> *o) {
+    if constexpr (BYTES == 8) {
+      const tbec_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+      o[0] = {__uint_as_float(v.x), __uint_as_float(v.y)};
+    } else {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const tbec_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * c, 0, 0);
-      if constexpr (sizeof(T) == 4) {
-        o[2 * c] = {__uint_as_float(v.x), __uint_as_float(v.y)};
-        o[2 * c + 1] = {__uint_as_float(v.z), __uint_as_float(v.w)};
-      } else {
-        o[c] = {__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z)};
+      for (int c = 0; c < NB; ++c) {
+        const tbec_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * c, 0, 0);
+        if constexpr (sizeof(T) == 4) {
+          o[2 * c] = {__uint_as_float(v.x), __uint_as_float(v.y)};
+          o[2 * c + 1] = {__uint_as_float(v.z), __uint_as_float(v.w)};
+        } else {
+          o[c] = {__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z)};
+        }
       }
     }
   }
   static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t off,
                                             const cplx<T> *o) {
+    if constexpr (BYTES == 8) {
+      const tbec_u32x2 v = tbec_u32x2{__float_as_uint(o[0].re), __float_as_uint(o[0].im)};
+      __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+    } else {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      tbec_u32x4 v;
-      if constexpr (sizeof(T) == 4) {
-        v = tbec_u32x4{__float_as_uint(o[2 * c].re), __float_as_uint(o[2 * c].im),
-                       __float_as_uint(o[2 * c + 1].re), __float_as_uint(o[2 * c + 1].im)};
-      } else {
-        v = tbec_u32x4{(unsigned)__double2loint(o[c].re), (unsigned)__double2hiint(o[c].re),
-                       (unsigned)__double2loint(o[c].im), (unsigned)__double2hiint(o[c].im)};
+      for (int c = 0; c < NB; ++c) {
+        tbec_u32x4 v;
+        if constexpr (sizeof(T) == 4) {
+          v = tbec_u32x4{__float_as_uint(o[2 * c].re), __float_as_uint(o[2 * c].im),
+                         __float_as_uint(o[2 * c + 1].re), __float_as_uint(o[2 * c + 1].im)};
+        } else {
+          v = tbec_u32x4{(unsigned)__double2loint(o[c].re), (unsigned)__double2hiint(o[c].re),
+                         (unsigned)__double2loint(o[c].im), (unsigned)__double2hiint(o[c].im)};
+        }
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off + 16 * c, 0, 0);
       }
-      __builtin_amdgcn_raw_buffer_store_b128(v, r, off + 16 * c, 0, 0);
     }
   }
 };

@@ -1,5 +1,5 @@
 // One (precision, tile) group of stage-kernel instantiations; compiled once per pair
-// with -DINST_PREC=32|64 -DINST_TILE=0..3.
+// with -DINST_PREC=32|64 -DINST_TILE=0..4.
 #include "stage_launch.hpp"
 
 #ifndef INST_PREC

@@ -14,10 +14,12 @@ template <> struct TileShape<32, 0> { static constexpr int TZ = 64, TY = 16, VZ 
 template <> struct TileShape<32, 1> { static constexpr int TZ = 64, TY = 32, VZ = 2; };
 template <> struct TileShape<32, 2> { static constexpr int TZ = 64, TY = 16, VZ = 2; };
 template <> struct TileShape<32, 3> { static constexpr int TZ = 32, TY = 32, VZ = 4; };
+template <> struct TileShape<32, 4> { static constexpr int TZ = 64, TY = 16, VZ = 1; };
 template <> struct TileShape<64, 0> { static constexpr int TZ = 32, TY = 16, VZ = 2; };
 template <> struct TileShape<64, 1> { static constexpr int TZ = 64, TY = 8, VZ = 2; };
 template <> struct TileShape<64, 2> { static constexpr int TZ = 32, TY = 16, VZ = 1; };
 template <> struct TileShape<64, 3> { static constexpr int TZ = 64, TY = 16, VZ = 1; };
+template <> struct TileShape<64, 4> { static constexpr int TZ = 32, TY = 8, VZ = 1; };
 
 #define TBEC_DECL_LAUNCHER(P, I) \
   hipError_t launch_stage_p##P##_t##I(const StageArgs &a, int sum, bool rot, bool imag, \
@@ -26,10 +28,12 @@ TBEC_DECL_LAUNCHER(32, 0)
 TBEC_DECL_LAUNCHER(32, 1)
 TBEC_DECL_LAUNCHER(32, 2)
 TBEC_DECL_LAUNCHER(32, 3)
+TBEC_DECL_LAUNCHER(32, 4)
 TBEC_DECL_LAUNCHER(64, 0)
 TBEC_DECL_LAUNCHER(64, 1)
 TBEC_DECL_LAUNCHER(64, 2)
 TBEC_DECL_LAUNCHER(64, 3)
+TBEC_DECL_LAUNCHER(64, 4)
 #undef TBEC_DECL_LAUNCHER
 
 #define TBEC_DECL_BLAUNCHER(P, I)                                                      \
@@ -39,10 +43,12 @@ TBEC_DECL_BLAUNCHER(32, 0)
 TBEC_DECL_BLAUNCHER(32, 1)
 TBEC_DECL_BLAUNCHER(32, 2)
 TBEC_DECL_BLAUNCHER(32, 3)
+TBEC_DECL_BLAUNCHER(32, 4)
 TBEC_DECL_BLAUNCHER(64, 0)
 TBEC_DECL_BLAUNCHER(64, 1)
 TBEC_DECL_BLAUNCHER(64, 2)
 TBEC_DECL_BLAUNCHER(64, 3)
+TBEC_DECL_BLAUNCHER(64, 4)
 #undef TBEC_DECL_BLAUNCHER
 
 // Dispatch to the (precision, tile) launcher.

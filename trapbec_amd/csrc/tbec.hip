@@ -52,9 +52,10 @@ int set_error(int code, const std::string &msg) {
 // ---------------------------------------------------------------- tile table ----
 // Each precision has a small set of tile shapes (TZ x TY x VZ); the segment length SX
 // is a runtime parameter.  The autotuner (trapbec_amd/autotune.py) picks (tile, SX).
-const TileCfg kTilesF32[] = {{64, 16, 4}, {64, 32, 2}, {64, 16, 2}, {32, 32, 4}};
-const TileCfg kTilesF64[] = {{32, 16, 2}, {64, 8, 2}, {32, 16, 1}, {64, 16, 1}};
-const int kNumTiles = 4;
+const TileCfg kTilesF32[] = {{64, 16, 4}, {64, 32, 2}, {64, 16, 2}, {32, 32, 4},
+                             {64, 16, 1}};
+const TileCfg kTilesF64[] = {{32, 16, 2}, {64, 8, 2}, {32, 16, 1}, {64, 16, 1}, {32, 8, 1}};
+const int kNumTiles = 5;
 
 const TileCfg &tile_cfg(int prec, int id) {
   return prec == TBEC_F32 ? kTilesF32[id] : kTilesF64[id];
@@ -80,14 +81,16 @@ hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool ro
       case 0: return launch_stage_p32_t0(a, sum, rot, imag, s);
       case 1: return launch_stage_p32_t1(a, sum, rot, imag, s);
       case 2: return launch_stage_p32_t2(a, sum, rot, imag, s);
-      default: return launch_stage_p32_t3(a, sum, rot, imag, s);
+      case 3: return launch_stage_p32_t3(a, sum, rot, imag, s);
+      default: return launch_stage_p32_t4(a, sum, rot, imag, s);
     }
   } else {
     switch (tile) {
       case 0: return launch_stage_p64_t0(a, sum, rot, imag, s);
       case 1: return launch_stage_p64_t1(a, sum, rot, imag, s);
       case 2: return launch_stage_p64_t2(a, sum, rot, imag, s);
-      default: return launch_stage_p64_t3(a, sum, rot, imag, s);
+      case 3: return launch_stage_p64_t3(a, sum, rot, imag, s);
+      default: return launch_stage_p64_t4(a, sum, rot, imag, s);
     }
   }
 }

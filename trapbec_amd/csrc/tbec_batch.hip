@@ -58,14 +58,16 @@ hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool 
       case 0: return launch_batch_p32_t0(L, sum, rot, grid, s);
       case 1: return launch_batch_p32_t1(L, sum, rot, grid, s);
       case 2: return launch_batch_p32_t2(L, sum, rot, grid, s);
-      default: return launch_batch_p32_t3(L, sum, rot, grid, s);
+      case 3: return launch_batch_p32_t3(L, sum, rot, grid, s);
+      default: return launch_batch_p32_t4(L, sum, rot, grid, s);
     }
   }
   switch (tile) {
     case 0: return launch_batch_p64_t0(L, sum, rot, grid, s);
     case 1: return launch_batch_p64_t1(L, sum, rot, grid, s);
     case 2: return launch_batch_p64_t2(L, sum, rot, grid, s);
-    default: return launch_batch_p64_t3(L, sum, rot, grid, s);
+    case 3: return launch_batch_p64_t3(L, sum, rot, grid, s);
+    default: return launch_batch_p64_t4(L, sum, rot, grid, s);
   }
 }
 }  // namespace tbec

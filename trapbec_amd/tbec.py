@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libtbec.so")
 
 TBEC_F32, TBEC_F64 = 32, 64
 SUM_REFERENCE, SUM_COMPENSATED, SUM_CENTRED = 0, 1, 2
-NUM_TILES = 4                   # tile shape ids 0..3 per precision
+NUM_TILES = 5                   # tile shape ids 0..4 per precision
 REAL_RK3, IMAG_RK3, IMAG_EULER_REF = 0, 1, 2
 ERRORS = {-1: "TBEC_ERR_ARG", -2: "TBEC_ERR_OOM", -3: "TBEC_ERR_HIP",
           -4: "TBEC_ERR_NONFINITE", -5: "TBEC_ERR_STATE"}
