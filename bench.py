@@ -215,6 +215,13 @@ def main():
         res.setdefault("stored", {})[prec] = v
         print(f"[bench] fp{prec} stored w: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
               flush=True)
+        if prec == 32:
+            # the same workload in a frame rotating at Omega = 0.7 (adds the D1 terms)
+            import dataclasses
+            v = run_gpu(args, prec, steps, spec, dataclasses.replace(pr, Omega=0.7))
+            res["rot"] = v
+            print(f"[bench] fp32 Omega=0.7: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
+                  flush=True)
         if prec == 32 and args.variants:
             for name in SUMS:
                 if name == args.fp32_sum:
@@ -267,6 +274,8 @@ def main():
         out["fp64"] = brief(res[64], args.fp64_steps)
     for prec, v in res.get("stored", {}).items():
         out[f"fp{prec}_stored_w"] = brief(v, args.steps if prec == 32 else args.fp64_steps)
+    if "rot" in res:
+        out["fp32_rotating_omega_0.7"] = brief(res["rot"], args.steps)
     for name, v in res.get("variants", {}).items():
         out.setdefault("fp32_variants", {})[name] = brief(v, args.steps)
     traffic = os.path.join(ROOT, "profiles", f"traffic_m{n}.json")
