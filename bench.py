@@ -88,7 +88,7 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False):
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
     c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
     c.set_w_storage(w_stored)
-    tile, seg, remap = autotune.lookup(pr.shape, prec, sm, w_stored)
+    tile, seg, remap = autotune.lookup(pr.shape, prec, sm, w_stored, pr.Omega != 0.0)
     if args.tile >= 0:
         tile = args.tile
     if args.seg > 0:

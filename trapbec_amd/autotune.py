@@ -17,10 +17,11 @@ DEFAULT = {32: (2, 0), 64: (2, 0)}      # seg 0: libtbec picks it from the grid 
 SEGS = (8, 16, 32, 64, 128, 256)
 
 
-def _key(shape, prec, sum_mode=None, w_stored=False):
+def _key(shape, prec, sum_mode=None, w_stored=False, rot=False):
     if sum_mode is None:
         sum_mode = 0 if prec == 64 else 2
-    return f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}:s{sum_mode}" + (":ws" if w_stored else "")
+    return (f"{shape[0]}x{shape[1]}x{shape[2]}:f{prec}:s{sum_mode}" +
+            (":ws" if w_stored else "") + (":rot" if rot else ""))
 
 
 def _load():
@@ -30,16 +31,17 @@ def _load():
     return {}
 
 
-def lookup(shape, prec, sum_mode=None, w_stored=False):
-    """(tile, seg, remap) recorded for this grid, precision, summation and w scheme."""
-    t = _load().get(_key(shape, prec, sum_mode, w_stored))
+def lookup(shape, prec, sum_mode=None, w_stored=False, rot=False):
+    """(tile, seg, remap) recorded for this grid, precision, summation, w scheme and
+    rotating frame (real time)."""
+    t = _load().get(_key(shape, prec, sum_mode, w_stored, rot))
     if t:
         return int(t["tile"]), int(t["seg"]), int(t.get("remap", 1))
     return DEFAULT[prec] + (1,)
 
 
 def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
-         table=None, w_stored=False, problem=None):
+         table=None, w_stored=False, problem=None, rot=False):
     """Time every (tile, seg) for this grid, precision, summation and w scheme (stages
     1-3 in turn); return the entry.  With `problem` (an initial.Problem) the stages of
     that problem are timed (rotation, imaginary time) and nothing is saved."""
@@ -52,7 +54,8 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
         c = tbec.Condensate.from_problem(problem, prec=prec, sum_mode=sum_mode)
     else:
         c = tbec.Condensate(shape, (16.0, 16.0, 16.0), (256.0,) * 3, c0=0.0, gnl=1000.0,
-                            dt=1e-4, prec=prec, sum_mode=sum_mode)
+                            dt=1e-4, prec=prec, sum_mode=sum_mode,
+                            omega_z=-1.4 if rot else 0.0)
     c.set_w_storage(w_stored)
     if psi0 is None:
         psi0 = np.full(shape, 0.1 + 0.05j)
@@ -76,7 +79,7 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
     if save:
         path = table or TABLE
         t = _load() if path == TABLE else (json.load(open(path)) if os.path.exists(path) else {})
-        t[_key(shape, prec, sum_mode, w_stored)] = entry
+        t[_key(shape, prec, sum_mode, w_stored, rot)] = entry
         with open(path, "w") as f:
             json.dump(t, f, indent=1, sort_keys=True)
     return entry
@@ -117,8 +120,10 @@ if __name__ == "__main__":
     ap.add_argument("--prec", type=int, default=32)
     ap.add_argument("--sum", type=int, default=None)
     ap.add_argument("--w-stored", action="store_true", help="the stored-w RK3 scheme")
+    ap.add_argument("--rot", action="store_true", help="rotating frame (Omega = 0.7)")
     ap.add_argument("--table", default=None, help="write here instead of tuning.json")
     a = ap.parse_args()
-    e = tune(tuple(a.shape), a.prec, a.sum, verbose=True, table=a.table, w_stored=a.w_stored)
+    e = tune(tuple(a.shape), a.prec, a.sum, verbose=True, table=a.table, w_stored=a.w_stored,
+             rot=a.rot)
     print(json.dumps({"shape": a.shape, "prec": a.prec, "sum": a.sum,
-                      "w_stored": a.w_stored, **e}), flush=True)
+                      "w_stored": a.w_stored, "rot": a.rot, **e}), flush=True)

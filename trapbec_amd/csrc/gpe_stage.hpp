@@ -537,8 +537,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   }
 }
 
+// Minimum waves per SIMD asked of the compiler.  512-thread FP32 real-time kernels are
+// held to 128 VGPRs so that two workgroups share a CU (the rotating variant needs 134
+// otherwise, which halves the resident waves); elsewhere the floor would cost spills.
+template <typename T, int NT, bool IMAG> constexpr int stage_min_waves() {
+  return (sizeof(T) == 4 && NT == 512 && !IMAG) ? 4 : 1;
+}
+
 template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
-__global__ void __launch_bounds__((TZ / VZ) * TY)
+__global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage(const StageArgs a) {
   constexpr int NT = (TZ / VZ) * TY;
   if constexpr (IMAG) {
