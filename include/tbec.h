@@ -120,6 +120,19 @@ int tbec_diagnose(tbec_handle *h, tbec_diag *out);
 int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
                     int check_every, tbec_solve_result *out);
 
+/* Output-time diagnostics of the current psi, as the reference's var_print writes
+ * them when vel_print / qpress_print are set (src/run.f90:64-66,
+ * src/output.f90:80-91).  Both fill 3*nx*ny*nz reals of the working precision,
+ * component slowest, then x, then y, z fastest (the memory order of the reference's
+ * vel(k,j,i,a) / qpressure(k,j,i,a)); host buffers; synchronise.
+ * tbec_velocity: get_phase (src/diagnostic.f90:27-46), the 6th-order gradient of
+ *   atan2(Im psi, Re psi), not unwrapped.
+ * tbec_qpressure: get_qpressure (src/diagnostic.f90:50-83), the 2nd-order gradient of
+ *   the 4th-order Laplacian of |psi|.
+ * Both use the handle's w array as scratch (w carries nothing across steps). */
+int tbec_velocity(tbec_handle *h, void *vel_host);
+int tbec_qpressure(tbec_handle *h, void *qp_host);
+
 /* Vortex cores of the current psi by phase winding around grid plaquettes, counting
  * plaquettes whose corners all have |psi|^2 >= dens_min.  orientation 0: z-normal
  * (x,y) plaquettes, 1: x-normal (y,z), 2: y-normal (z,x).  Writes up to cap entries

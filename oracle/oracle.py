@@ -55,6 +55,8 @@ def lib():
             getattr(L, "orc_rk3_step" + suf).restype = C.c_double
             getattr(L, "orc_euler_step" + suf).argtypes = [P, vp, vp, C.c_double]
             getattr(L, "orc_psi_info" + suf).argtypes = [P, vp, vp]
+            getattr(L, "orc_velocity" + suf).argtypes = [P, vp, vp]
+            getattr(L, "orc_qpressure" + suf).argtypes = [P, vp, vp]
         L.orc_sums_f64.argtypes = [P, vp, vp, vp]
         L.orc_energy_f64.argtypes = [P, vp, vp, C.c_double, vp]
         L.orc_imag_rk3_step_f64.argtypes = [P, vp, vp, vp, C.c_double, vp, vp]
@@ -125,6 +127,18 @@ class Stepper:
     def info(self) -> np.ndarray:
         out = np.zeros(4)
         getattr(lib(), "orc_psi_info" + self.suf)(C.byref(self.p), _ptr(self.psi), _ptr(out))
+        return out
+
+    def velocity(self) -> np.ndarray:
+        """get_phase's velocity field, indexed [component, x, y, z]."""
+        out = np.zeros((3,) + self.rhs.shape, dtype=self.psi.real.dtype)
+        getattr(lib(), "orc_velocity" + self.suf)(C.byref(self.p), _ptr(self.psi), _ptr(out))
+        return out
+
+    def qpressure(self) -> np.ndarray:
+        """get_qpressure's field, indexed [component, x, y, z]."""
+        out = np.zeros((3,) + self.rhs.shape, dtype=self.psi.real.dtype)
+        getattr(lib(), "orc_qpressure" + self.suf)(C.byref(self.p), _ptr(self.psi), _ptr(out))
         return out
 
     # FP64-only quantities with no reference counterpart

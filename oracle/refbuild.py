@@ -43,7 +43,7 @@ DEFAULTS = dict(
     potential="harmonic", harm_wx=256.0, harm_wy=256.0, harm_wz=256.0,
     potential_off=None, gamma=0.0, two_dim=False, omega_z=0.0, advective_vx=0.0,
     imaginary_time=False, imag_nsteps=1, full_binary_output=True,
-    imaginary_output=False,
+    imaginary_output=False, vel_print=False, qpress_print=False,
 )
 
 
@@ -101,8 +101,8 @@ def cparam_text(p: dict) -> str:
         f"logical, parameter :: full_binary_output={_lg(p['full_binary_output'])}",
         f"logical, parameter :: imaginary_output={_lg(p['imaginary_output'])}",
         "logical, parameter :: gnuplot_2d_print=.false.",
-        "logical, parameter :: qpress_print=.false.",
-        "logical, parameter :: vel_print=.false.",
+        f"logical, parameter :: qpress_print={_lg(p['qpress_print'])}",
+        f"logical, parameter :: vel_print={_lg(p['vel_print'])}",
         "",
     ])
 
@@ -220,6 +220,26 @@ def run(d: str, psi0, prec: int = 64, nprocs: int = 1, itime0: int = 1,
         out[cx * mx:(cx + 1) * mx, cy * my:(cy + 1) * my, :] = blk
         t_out = float(hdr[3])
     return {"psi": out, "t": t_out, "stdout": res.stdout}
+
+
+def read_field(d: str, prefix: str, shape, nprocs: int = 1, prec: int = 64, shot: int = 1
+               ) -> np.ndarray:
+    """A vector field written by var_print (``src/output.f90:80-91``: ``vel%02d.dat`` /
+    ``qpress%02d.dat``, one record ``coords(2) int4, field(nz, ny, nx, 3)``), assembled
+    over ranks as an array indexed [component, x, y, z]."""
+    rdt = np.float64 if prec == 64 else np.float32
+    nx, ny, nz = shape
+    dx, dy = _dims_create(nprocs)
+    mx, my = nx // dx, ny // dy
+    out = np.zeros((3, nx, ny, nz), dtype=rdt)
+    for r in range(nprocs):
+        rec = _read_records(os.path.join(d, "data", f"proc{r:03d}",
+                                         f"{prefix}{shot:02d}.dat"))[0]
+        coords = np.frombuffer(rec[:8], dtype=np.int32)
+        blk = np.frombuffer(rec[8:], dtype=rdt).reshape(3, mx, my, nz)
+        cx, cy = int(coords[0]), int(coords[1])
+        out[:, cx * mx:(cx + 1) * mx, cy * my:(cy + 1) * my, :] = blk
+    return out
 
 
 if __name__ == "__main__":

@@ -38,21 +38,25 @@ typedef struct {
 #define T double
 #define SUF _f64
 #define HYPOT hypot
+#define ATAN2 atan2
 #define FLT_EPSILON_T DBL_EPSILON
 #include "tbec_oracle_impl.h"
 #undef T
 #undef SUF
 #undef HYPOT
+#undef ATAN2
 #undef FLT_EPSILON_T
 
 #define T float
 #define SUF _f32
 #define HYPOT hypotf
+#define ATAN2 atan2f
 #define FLT_EPSILON_T FLT_EPSILON
 #include "tbec_oracle_impl.h"
 #undef T
 #undef SUF
 #undef HYPOT
+#undef ATAN2
 #undef FLT_EPSILON_T
 
 /* ---------------- Part 2: FP64 quantities with no reference counterpart --------- */

@@ -238,6 +238,83 @@ void FN(orc_psi_info)(const orc_params *p, const T *psi_pad, double out[4]) {
   out[3] = (double)s2;
 }
 
+/* get_phase, src/diagnostic.f90:27-46: phase = atan2(Im psi, Re psi) over the whole
+ * padded grid (zero ghosts give phase 0), then vel(k,j,i,a) = first_deriv of the phase
+ * along axis a (src/deriv.f90:5-11: 6th order, real arithmetic, no unwrapping, so a
+ * 2*pi jump in the phase shows up as a large velocity).  vel_out: 3*mx*my*mz values,
+ * component slowest, then x, y, z fastest (the reference's vel(k,j,i,a) order). */
+void FN(orc_velocity)(const orc_params *p, const T *psi_pad, T *vel_out) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6), np = (long)(mx + 6) * sx;
+  const T hx = (T)p->Lx / (T)p->nx, hy = (T)p->Ly / (T)p->ny, hz = (T)p->Lz / (T)p->nz;
+  const FN(cplx) *P = (const FN(cplx) *)psi_pad;
+  T *ph = (T *)malloc(sizeof(T) * np);
+  for (long q = 0; q < np; ++q) ph[q] = ATAN2(P[q].im, P[q].re);
+  const long n = (long)mx * my * mz;
+  const long st[3] = {sx, sy, 1};
+  const T h[3] = {hx, hy, hz};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < mx; ++i)
+    for (int j = 0; j < my; ++j)
+      for (int k = 0; k < mz; ++k) {
+        const T *c = ph + (long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3);
+        const long o = ((long)i * my + j) * mz + k;
+        for (int a = 0; a < 3; ++a) {
+          const long s = st[a];
+          const T v = ((((-c[-3 * s] + (T)9 * c[-2 * s]) - (T)45 * c[-s]) + (T)45 * c[s]) -
+                       (T)9 * c[2 * s]) + c[3 * s];
+          vel_out[a * n + o] = v / ((T)60 * h[a]);
+        }
+      }
+  free(ph);
+}
+
+/* get_qpressure, src/diagnostic.f90:50-83: density = abs(psi) over the padded grid,
+ * laplace_rho on the box extended by one cell, 4th order (real_second_deriv,
+ * src/deriv.f90:26-32, summed x + y + z), then qpressure(k,j,i,a) =
+ * low_first_deriv of laplace_rho along a (src/deriv.f90:33-39, 2nd order).  Output
+ * layout as orc_velocity. */
+void FN(orc_qpressure)(const orc_params *p, const T *psi_pad, T *qp_out) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6), np = (long)(mx + 6) * sx;
+  const T hx = (T)p->Lx / (T)p->nx, hy = (T)p->Ly / (T)p->ny, hz = (T)p->Lz / (T)p->nz;
+  const FN(cplx) *P = (const FN(cplx) *)psi_pad;
+  T *dn = (T *)malloc(sizeof(T) * np);
+  T *lp = (T *)calloc((size_t)np, sizeof(T));
+  for (long q = 0; q < np; ++q) dn[q] = HYPOT(P[q].re, P[q].im);
+  const long st[3] = {sx, sy, 1};
+  const T h[3] = {hx, hy, hz};
+#pragma omp parallel for schedule(static)
+  for (int i = -1; i <= mx; ++i)
+    for (int j = -1; j <= my; ++j)
+      for (int k = -1; k <= mz; ++k) {
+        const long o = (long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3);
+        const T *c = dn + o;
+        T d[3];
+        for (int a = 0; a < 3; ++a) {
+          const long s = st[a];
+          const T v = ((((-c[-2 * s] + (T)16 * c[-s]) - (T)30 * c[0]) + (T)16 * c[s]) -
+                       c[2 * s]);
+          d[a] = v / ((T)12 * (h[a] * h[a]));
+        }
+        lp[o] = (d[0] + d[1]) + d[2];
+      }
+  const long n = (long)mx * my * mz;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < mx; ++i)
+    for (int j = 0; j < my; ++j)
+      for (int k = 0; k < mz; ++k) {
+        const T *c = lp + (long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3);
+        const long o = ((long)i * my + j) * mz + k;
+        for (int a = 0; a < 3; ++a) {
+          const long s = st[a];
+          qp_out[a * n + o] = (-c[-s] + c[s]) / ((T)2 * h[a]);
+        }
+      }
+  free(dn);
+  free(lp);
+}
+
 #undef FN
 #undef CAT
 #undef CAT_

@@ -88,6 +88,26 @@ def case_c1(nprocs=4, steps=1000, seed=20261203):
     return wall
 
 
+def case_diag(name, prec, nprocs=1, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), seed=13):
+    """The output-time diagnostics (src/run.f90:64-66, src/diagnostic.f90:27-84): one
+    real-time step from a seeded state with two vortex lines, then the reference's
+    velocity (6th-order gradient of the unwrapped-nowhere phase) and quantum-pressure
+    (2nd-order gradient of the 4th-order Laplacian of |psi|) records."""
+    harm = tuple(L * L for L in box)
+    d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                       Lx=box[0], Ly=box[1], Lz=box[2], nsteps=1, shots=1,
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2],
+                       vel_print=True, qpress_print=True)
+    psi0 = small_ic(shape, box, seed)
+    psi0 = psi0 * initial.vortex_line_factor(shape, box, -2.1, 1.7, -1, 1 / math.sqrt(10))
+    psi0 = psi0.astype(np.complex128 if prec == 64 else np.complex64)
+    r = refbuild.run(d, psi0, prec=prec, nprocs=nprocs)
+    vel = refbuild.read_field(d, "vel", shape, nprocs, prec)
+    qp = refbuild.read_field(d, "qpress", shape, nprocs, prec)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi=r["psi"], vel=vel, qpress=qp,
+                        prec=prec, box=np.array(box), nprocs=nprocs)
+
+
 CASES = {
     "rk3_f64_16": lambda: case_small("rk3_f64_16", 64),
     "rk3_f64_16_rot": lambda: case_small("rk3_f64_16_rot", 64, omega_z=-1.4),
@@ -99,6 +119,9 @@ CASES = {
                                         box=(16.0, 12.0, 10.0), omega_z=-0.8),
     "rk3_f64_16_np4": lambda: case_small("rk3_f64_16_np4", 64, omega_z=-1.4, nprocs=4),
     "euler_f64_16": case_euler,
+    "diag_f64_16": lambda: case_diag("diag_f64_16", 64),
+    "diag_f32_16": lambda: case_diag("diag_f32_16", 32),
+    "diag_f64_16_np4": lambda: case_diag("diag_f64_16_np4", 64, nprocs=4),
     "c1_f64_64": case_c1,
 }
 

@@ -312,3 +312,85 @@ def test_vortex_plaquettes_ring_orientations():
     xx = x[vx[:, 0]]
     rho = np.sqrt(xx ** 2 + yy ** 2)
     assert np.all(np.abs(rho - 3.0) < 0.5)
+
+
+@pytest.mark.parametrize("name", ["diag_f64_16", "diag_f32_16"])
+def test_diagnostics_match_reference_bitwise(name):
+    """get_phase's velocity and get_qpressure's field (src/diagnostic.f90:27-83), as the
+    reference wrote them (vel01.dat, qpress01.dat), bit for bit from the same psi."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    prec = int(f["prec"])
+    box = tuple(float(b) for b in f["box"])
+    s = oracle.Stepper(oracle.params(f["psi"].shape, box, tuple(L * L for L in box)),
+                       f["psi"], prec=prec)
+    assert np.array_equal(s.velocity(), f["vel"])
+    assert np.array_equal(s.qpressure(), f["qpress"])
+
+
+def test_diagnostics_four_ranks():
+    """On 2 x 2 ranks the reference's velocity equals its one-rank field.  Its quantum
+    pressure does not: ghost_comm fills face ghosts only (src/ghost.f90:27-192), and
+    the Laplacian on the one-cell-extended box reads the unfilled corner ghosts next
+    to inner rank boundaries.  Rebuilding each rank's padded block that way (faces from
+    the neighbours, corners zero) reproduces the 4-rank file bit for bit; the single
+    GPU domain has no such corners and matches the one-rank file."""
+    one = np.load(os.path.join(GOLDEN, "diag_f64_16.npz"))
+    four = np.load(os.path.join(GOLDEN, "diag_f64_16_np4.npz"))
+    assert np.array_equal(four["psi"], one["psi"])
+    assert np.array_equal(four["vel"], one["vel"])
+    assert not np.array_equal(four["qpress"], one["qpress"])
+    psi = one["psi"]
+    nx, ny, nz = psi.shape
+    box = tuple(float(b) for b in one["box"])
+    full = oracle.pad(psi)
+    mx, my = nx // 2, ny // 2
+    qp = np.zeros_like(one["qpress"])
+    for cx in range(2):
+        for cy in range(2):
+            x0, y0 = cx * mx, cy * my
+            blk = np.zeros((mx + 6, my + 6, nz + 6), dtype=np.complex128)
+            # interior plus x faces (all interior y) plus y faces (all interior x)
+            blk[:, 3:-3, :] = full[x0:x0 + mx + 6, y0 + 3:y0 + 3 + my, :]
+            blk[3:-3, :, :] = full[x0 + 3:x0 + 3 + mx, y0:y0 + my + 6, :]
+            p = oracle.params((nx, ny, nz), box, tuple(L * L for L in box), dims=(2, 2),
+                              coords=(cx, cy))
+            s = oracle.Stepper(p, np.zeros((mx, my, nz)))
+            s.psi = blk
+            qp[:, x0:x0 + mx, y0:y0 + my, :] = s.qpressure()
+    assert np.array_equal(qp, four["qpress"])
+
+
+def test_diagnostics_closed_forms():
+    """Velocity of a plane wave exp(i k x) is k away from the phase jumps (6th order);
+    the quantum-pressure gradient of a Gaussian |psi| matches its analytic form to the
+    stencils' order."""
+    n, L = 48, 12.0
+    shape = (n, n, n)
+    X, Y, Z = initial.mesh(shape, (L, L, L))
+    kx = 2 * math.pi / L * 2
+    psi = np.exp(1j * kx * X) * np.exp(-0.02 * (X ** 2 + Y ** 2 + Z ** 2))
+    s = oracle.Stepper(oracle.params(shape, L, (1.0, 1.0, 1.0)), psi)
+    v = s.velocity()
+    ph = np.angle(psi)
+    h = L / n
+    smooth = np.ones(shape, bool)
+    for sh in range(-3, 4):   # no 2*pi jump inside the x stencil
+        smooth &= np.abs(np.roll(ph, sh, 0) - ph) < math.pi
+    smooth[:3] = smooth[-3:] = False
+    assert np.abs(v[0][smooth] - kx).max() < 1e-6
+    assert np.abs(v[1][3:-3, 3:-3, 3:-3]).max() < 1e-10   # phase independent of y
+    a = 0.3
+    errs = []
+    for m in (48, 96):
+        Xm, Ym, Zm = initial.mesh((m, m, m), (L, L, L))
+        r2 = Xm ** 2 + Ym ** 2 + Zm ** 2
+        rho = np.exp(-a * r2)                                  # |psi| itself
+        s = oracle.Stepper(oracle.params((m, m, m), L, (1.0, 1.0, 1.0)), rho.astype(complex))
+        q = s.qpressure()
+        # lap(rho) = (4 a^2 r^2 - 6 a) rho; its x derivative:
+        dlap = (8 * a * a * Xm - 2 * a * Xm * (4 * a * a * r2 - 6 * a)) * rho
+        inner = (np.abs(Xm) < 3) & (np.abs(Ym) < 3) & (np.abs(Zm) < 3)
+        errs.append(np.abs(q[0][inner] - dlap[inner]).max() / np.abs(dlap[inner]).max())
+    assert errs[1] < 1e-2
+    assert 3.5 < errs[0] / errs[1] < 4.5          # second order (low_first_deriv)
+    assert h < 0.3

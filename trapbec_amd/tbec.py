@@ -71,7 +71,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
            "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
-           "tbec_set_w_storage"]
+           "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure"]
 
 _lib = None
 
@@ -89,7 +89,7 @@ def load(path: str = LIB_PATH):
     L.tbec_create.argtypes = [C.POINTER(TbecParams), C.POINTER(H)]
     L.tbec_destroy.argtypes = [H]
     for f in ("tbec_upload_psi", "tbec_download_psi", "tbec_upload_psi_device",
-              "tbec_download_psi_device", "tbec_rhs"):
+              "tbec_download_psi_device", "tbec_rhs", "tbec_velocity", "tbec_qpressure"):
         getattr(L, f).argtypes = [H, vp]
     L.tbec_step.argtypes = [H, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     L.tbec_diagnose.argtypes = [H, C.POINTER(TbecDiag)]
@@ -225,6 +225,21 @@ class Condensate:
         _check(load().tbec_imag_solve(self._h, max_steps, tol, norm, check_every,
                                       C.byref(r)))
         return {k: getattr(r, k) for k, _ in TbecSolveResult._fields_ if k != "pad_"}
+
+    def _rdt(self):
+        return np.float64 if self.cdt == np.complex128 else np.float32
+
+    def velocity(self) -> np.ndarray:
+        """The reference's velocity output field (get_phase), [component, x, y, z]."""
+        out = np.empty((3,) + tuple(self.shape), dtype=self._rdt())
+        _check(load().tbec_velocity(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def qpressure(self) -> np.ndarray:
+        """The reference's quantum-pressure output field, [component, x, y, z]."""
+        out = np.empty((3,) + tuple(self.shape), dtype=self._rdt())
+        _check(load().tbec_qpressure(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def vortex_cores(self, dens_min: float, orientation: int = 0, cap: int = 1 << 20):
         """(i, j, k, charge) of every plaquette with nonzero phase winding."""
