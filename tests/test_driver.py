@@ -123,3 +123,26 @@ def test_driver_restart_is_bitwise(tmp_path):
     _, hb, pb = _read_var(b / "data" / "proc000" / "var0002.dat", (16, 16, 16))
     assert np.array_equal(pa, pb)
     assert ha[3] == hb[3]
+
+
+@pytest.mark.gpu
+def test_driver_writes_velocity_and_qpressure(tmp_path):
+    """vel_print / qpress_print write vel01.dat and qpress01.dat (src/output.f90:80-91:
+    coords, field(nz, ny, nx, 3)) from the state of var0001.dat; they equal the oracle's
+    restatement of get_phase / get_qpressure on that state."""
+    _need_driver()
+    nml = tmp_path / "run.nml"
+    _nml(nml, **C1_16, nsteps=5, shots=5, vel_print=True, qpress_print=True,
+         out_dir=str(tmp_path / "data"))
+    print(f"\n[case] driver vel/qpress 16^3", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = tmp_path / "data" / "proc000"
+    _, _, psi = _read_var(d / "var0001.dat", (16, 16, 16))
+    s = oracle.Stepper(oracle.params((16, 16, 16), 16.0, (256.0,) * 3), psi)
+    for name, ref in (("vel", s.velocity()), ("qpress", s.qpressure())):
+        rec = _read_records(d / f"{name}01.dat")[0]
+        assert list(np.frombuffer(rec[:8], np.int32)) == [0, 0]
+        got = np.frombuffer(rec[8:], np.float64).reshape(3, 16, 16, 16)
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), name

@@ -53,6 +53,14 @@ module tbec_iface
       import :: c_int, c_ptr
       type(c_ptr), value :: h, psi
     end function
+    integer(c_int) function tbec_velocity(h, vel) bind(C, name="tbec_velocity")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, vel
+    end function
+    integer(c_int) function tbec_qpressure(h, qp) bind(C, name="tbec_qpressure")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, qp
+    end function
     integer(c_int) function tbec_step(h, nsteps, mode, t) bind(C, name="tbec_step")
       import :: c_int, c_ptr, c_int64_t, c_double
       type(c_ptr), value :: h

@@ -36,6 +36,7 @@ program tbec_run
   integer(int64) :: seed = 20261203_int64
   real(dp) :: noise_amp = 0.0_dp, phase_noise_amp = 0.0_dp
   logical :: imaginary_time = .false., ic_only = .false., full_binary_output = .true.
+  logical :: vel_print = .false., qpress_print = .false.
   integer :: imag_nsteps = 0, imag_check = 50
   real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
   character(len=256) :: out_dir = 'data'
@@ -43,13 +44,16 @@ program tbec_run
     harm_wx, harm_wy, harm_wz, omega_z, c0, gnl, prec, sum_mode, tile, init_cond, &
     trap_omega, mu, g_phys, tf_norm, psi_scale, n_lines, line_x, line_y, line_q, xi, &
     ring_radius, seed, noise_amp, phase_noise_amp, imaginary_time, ic_only, &
-    full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir
+    full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir, &
+    vel_print, qpress_print
 
   character(len=512) :: nml_file
   integer :: u, ios, itime, nstart, l
   real(dp), allocatable :: xx(:), yy(:), zz(:)
   complex(dp), allocatable, target :: psi(:, :, :)
   complex(real32), allocatable, target :: psi4(:, :, :)
+  real(dp), allocatable, target :: fld(:, :, :, :)
+  real(real32), allocatable, target :: fld4(:, :, :, :)
   type(tbec_params) :: p
   type(tbec_diag) :: d
   type(tbec_solve_result) :: sr
@@ -139,6 +143,8 @@ program tbec_run
       print *, itime / shots, t, d%max_abs, d%min_abs, d%mean_abs, d%int_psi2
       call download()
       call write_var(itime / shots)
+      if (vel_print) call write_field('vel', itime / shots)
+      if (qpress_print) call write_field('qpress', itime / shots)
     end if
   end do
   call check(tbec_sync(h))
@@ -242,6 +248,40 @@ contains
     else
       write(uv) real(t, real32)
       write(uv) cmplx(psi, kind=real32)
+    end if
+    close(uv)
+  end subroutine
+
+  ! vel%02d.dat / qpress%02d.dat: one record coords(2), field(nz, ny, nx, 3) in the run's
+  ! precision (src/output.f90:80-91), computed on the device from the current psi.
+  subroutine write_field(name, shot)
+    character(len=*), intent(in) :: name
+    integer, intent(in) :: shot
+    character(len=512) :: fname
+    integer :: uv
+    integer(4) :: crd(2)
+    crd = 0
+    if (prec == 64) then
+      if (.not. allocated(fld)) allocate(fld(Nz, Ny, Nx, 3))
+      if (name == 'vel') then
+        call check(tbec_velocity(h, c_loc(fld)))
+      else
+        call check(tbec_qpressure(h, c_loc(fld)))
+      end if
+    else
+      if (.not. allocated(fld4)) allocate(fld4(Nz, Ny, Nx, 3))
+      if (name == 'vel') then
+        call check(tbec_velocity(h, c_loc(fld4)))
+      else
+        call check(tbec_qpressure(h, c_loc(fld4)))
+      end if
+    end if
+    write(fname, '(a,a,a,i2.2,a)') trim(out_dir), '/proc000/', name, shot, '.dat'
+    open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
+    if (prec == 64) then
+      write(uv) crd, fld
+    else
+      write(uv) crd, fld4
     end if
     close(uv)
   end subroutine
