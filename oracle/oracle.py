@@ -28,7 +28,10 @@ class OrcParams(C.Structure):
                 ("harm_w", C.c_double * 3), ("omega_z", C.c_double),
                 ("c0", C.c_double), ("gnl", C.c_double),
                 ("dt_re", C.c_double), ("dt_im", C.c_double),
-                ("potential_off", C.c_double), ("t_spindown", C.c_double)]
+                ("potential_off", C.c_double), ("t_spindown", C.c_double),
+                ("boundaries", C.c_int), ("periodicz", C.c_int)]
+
+BOUNDARIES = {"zero": 0, "solid": 1, "periodic": 2}
 
 
 def build(force: bool = False) -> str:
@@ -67,7 +70,8 @@ def lib():
 
 
 def params(shape, L, harm_w, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3, gamma=0.0,
-           dims=(1, 1), coords=(0, 0), two_dim=False, potential_off=None):
+           dims=(1, 1), coords=(0, 0), two_dim=False, potential_off=None,
+           boundaries="zero", periodicz=False):
     """Reference-unit parameters.  dt may be complex; the reference multiplies it by
     (1 - i*gamma) before the real-time loop (src/run.f90:179)."""
     nx, ny, nz = shape
@@ -77,7 +81,7 @@ def params(shape, L, harm_w, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3, gamma=0.0,
                   Lx, Ly, Lz, (C.c_double * 3)(*harm_w), omega_z, c0, gnl,
                   dtc.real, dtc.imag,
                   np.finfo(np.float64).max if potential_off is None else potential_off,
-                  np.finfo(np.float64).max)
+                  np.finfo(np.float64).max, BOUNDARIES[boundaries], int(periodicz))
     return p
 
 

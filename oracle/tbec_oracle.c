@@ -33,6 +33,8 @@ typedef struct {
   double c0, gnl;        /* (c0 - gnl*|psi|^2 - pot); both 1 in the reference */
   double dt_re, dt_im;   /* complex dt after the (1 - i*gamma) factor */
   double potential_off, t_spindown;
+  int boundaries;        /* 0 zero, 1 solid (mirror about the edge point), 2 periodic */
+  int periodicz;         /* z periodic whatever the boundaries */
 } orc_params;
 
 #define T double
@@ -90,7 +92,7 @@ void orc_sums_f64(const orc_params *p, const double *psi_pad, const double *rhs,
 }
 
 /* Energy diagnostics from one get_rhs evaluation: out = {N, E, mu, residual}. */
-void orc_energy_f64(const orc_params *p, const double *psi_pad, double *rhs_scratch,
+void orc_energy_f64(const orc_params *p, double *psi_pad, double *rhs_scratch,
                     double t, double out[4]) {
   double s[4];
   orc_get_rhs_f64(p, psi_pad, rhs_scratch, t);

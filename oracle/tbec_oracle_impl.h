@@ -76,13 +76,54 @@ static inline FN(cplx) FN(d1)(const FN(cplx) *v, long s, T h) {
   return FN(cdiv_real)(r, den);
 }
 
+/* ghost_comm, src/ghost.f90:27-192, on one domain (or the global boundary faces of a
+ * rank whose inner faces the caller filled): faces only, over the interior of the other
+ * two axes; corners are left as they are.  zero: ghosts 0; solid: mirror about the
+ * edge point (ghost 0 = psi 2, -1 = 3, -2 = 4); periodic: the opposite side (ghost 0 =
+ * psi n, n+1 = 1, ...).  z follows `boundaries` unless periodicz.  With more than one
+ * rank only the global faces are written, which is complete for zero and solid. */
+void FN(orc_fill_ghosts)(const orc_params *p, T *psi_pad) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6);
+  FN(cplx) *P = (FN(cplx) *)psi_pad;
+  const int bc = p->boundaries;
+  const int bz = p->periodicz ? 2 : bc;
+  const int n[3] = {mx, my, mz};
+  const long st[3] = {sx, sy, 1};
+  const int lo_edge[3] = {p->coord_x == 0, p->coord_y == 0, 1};
+  const int hi_edge[3] = {p->coord_x == p->dims_x - 1, p->coord_y == p->dims_y - 1, 1};
+  for (int a = 0; a < 3; ++a) {
+    const int b = a == 2 ? bz : bc;
+    const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;   /* the two other axes */
+    for (int iu = 1; iu <= n[u]; ++iu)
+      for (int iv = 1; iv <= n[v]; ++iv) {
+        FN(cplx) *row = P + (long)(iu + 2) * st[u] + (long)(iv + 2) * st[v] + 2 * st[a];
+        /* row[q * st[a]] is the point with index q (1..n) along axis a */
+        for (int g = 1; g <= 3; ++g) {
+          FN(cplx) zero = {(T)0, (T)0};
+          FN(cplx) lo = zero, hi = zero;
+          if (b == 1) {
+            lo = row[(1 + g) * st[a]];
+            hi = row[(n[a] - g) * st[a]];
+          } else if (b == 2) {
+            lo = row[(n[a] + 1 - g) * st[a]];
+            hi = row[g * st[a]];
+          }
+          if (lo_edge[a] || a == 2) row[(1 - g) * st[a]] = lo;
+          if (hi_edge[a] || a == 2) row[(n[a] + g) * st[a]] = hi;
+        }
+      }
+  }
+}
+
 /* get_rhs, src/finite_diff.f90:8-136, for the harmonic potential (:33-34), the
  * rotating frame (:128-130) and the nonlinearity (:132), with zero ghosts
  * (src/ghost.f90:55,88,121,154,179-181: the padded halo is never written).
  * Generalised by two coefficients that equal 1 in the reference:
  *   rhs += (c0 - gnl*abs(psi**2) - pot)*psi
  * (gnl*x == x exactly when gnl == 1, so the reference arithmetic is unchanged). */
-void FN(orc_get_rhs)(const orc_params *p, const T *psi_pad, T *rhs_out, double t) {
+void FN(orc_get_rhs)(const orc_params *p, T *psi_pad, T *rhs_out, double t) {
+  FN(orc_fill_ghosts)(p, psi_pad);
   const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
   const long sz = 1, sy = mz + 6, sx = (long)(my + 6) * (mz + 6);
   const T hx = (T)p->Lx / (T)p->nx, hy = (T)p->Ly / (T)p->ny, hz = (T)p->Lz / (T)p->nz;
@@ -243,7 +284,8 @@ void FN(orc_psi_info)(const orc_params *p, const T *psi_pad, double out[4]) {
  * along axis a (src/deriv.f90:5-11: 6th order, real arithmetic, no unwrapping, so a
  * 2*pi jump in the phase shows up as a large velocity).  vel_out: 3*mx*my*mz values,
  * component slowest, then x, y, z fastest (the reference's vel(k,j,i,a) order). */
-void FN(orc_velocity)(const orc_params *p, const T *psi_pad, T *vel_out) {
+void FN(orc_velocity)(const orc_params *p, T *psi_pad, T *vel_out) {
+  FN(orc_fill_ghosts)(p, psi_pad);
   const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
   const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6), np = (long)(mx + 6) * sx;
   const T hx = (T)p->Lx / (T)p->nx, hy = (T)p->Ly / (T)p->ny, hz = (T)p->Lz / (T)p->nz;
@@ -274,7 +316,8 @@ void FN(orc_velocity)(const orc_params *p, const T *psi_pad, T *vel_out) {
  * src/deriv.f90:26-32, summed x + y + z), then qpressure(k,j,i,a) =
  * low_first_deriv of laplace_rho along a (src/deriv.f90:33-39, 2nd order).  Output
  * layout as orc_velocity. */
-void FN(orc_qpressure)(const orc_params *p, const T *psi_pad, T *qp_out) {
+void FN(orc_qpressure)(const orc_params *p, T *psi_pad, T *qp_out) {
+  FN(orc_fill_ghosts)(p, psi_pad);
   const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
   const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6), np = (long)(mx + 6) * sx;
   const T hx = (T)p->Lx / (T)p->nx, hy = (T)p->Ly / (T)p->ny, hz = (T)p->Lz / (T)p->nz;

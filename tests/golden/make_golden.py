@@ -37,18 +37,27 @@ def small_ic(shape, box, seed, scale=True):
 
 
 def case_small(name, prec, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), steps=20,
-               omega_z=0.0, gamma=0.0, dt=1e-3 + 0j, nprocs=1, seed=11):
+               omega_z=0.0, gamma=0.0, dt=1e-3 + 0j, nprocs=1, seed=11, boundaries="zero",
+               periodicz=False, offset=(0.0, 0.0)):
     harm = tuple(L * L for L in box)   # pot = x^2 + y^2 + z^2, i.e. 2V with w = 1
     d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
                        Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=steps,
                        omega_z=omega_z, gamma=gamma, dt_re=dt.real, dt_im=dt.imag,
-                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
-    psi0 = small_ic(shape, box, seed).astype(np.complex128 if prec == 64 else np.complex64)
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2],
+                       boundaries=boundaries, periodicz=periodicz)
+    psi0 = small_ic(shape, box, seed)
+    if boundaries != "zero" or periodicz:
+        # a state that is not small at the walls, so the ghost values matter
+        X, Y, Z = initial.mesh(shape, box)
+        psi0 = psi0 + 3.0 * np.exp(1j * (0.7 * X - 0.4 * Y + 0.9 * Z + offset[0])) * \
+            np.cos(0.3 * Y + offset[1])
+    psi0 = psi0.astype(np.complex128 if prec == 64 else np.complex64)
     r = refbuild.run(d, psi0, prec=prec, nprocs=nprocs)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), psi0=psi0, psi=r["psi"],
                         t=r["t"], steps=steps, prec=prec, box=np.array(box),
                         harm_w=np.array(harm), omega_z=omega_z, gamma=gamma,
-                        dt=np.complex128(dt), nprocs=nprocs)
+                        dt=np.complex128(dt), nprocs=nprocs, boundaries=boundaries,
+                        periodicz=periodicz)
     return r
 
 
@@ -118,6 +127,19 @@ CASES = {
     "rk3_f64_aniso": lambda: case_small("rk3_f64_aniso", 64, shape=(16, 12, 20),
                                         box=(16.0, 12.0, 10.0), omega_z=-0.8),
     "rk3_f64_16_np4": lambda: case_small("rk3_f64_16_np4", 64, omega_z=-1.4, nprocs=4),
+    "rk3_f64_16_solid": lambda: case_small("rk3_f64_16_solid", 64, omega_z=-1.4,
+                                           boundaries="solid"),
+    "rk3_f32_16_solid": lambda: case_small("rk3_f32_16_solid", 32, boundaries="solid"),
+    "rk3_f64_16_periodic": lambda: case_small("rk3_f64_16_periodic", 64, omega_z=-1.4,
+                                              boundaries="periodic"),
+    "rk3_f64_16_periodic_np4": lambda: case_small("rk3_f64_16_periodic_np4", 64,
+                                                  omega_z=-1.4, boundaries="periodic",
+                                                  nprocs=4),
+    "rk3_f64_aniso_periodicz": lambda: case_small("rk3_f64_aniso_periodicz", 64,
+                                                  shape=(16, 12, 20), box=(16.0, 12.0, 10.0),
+                                                  omega_z=-0.8, periodicz=True),
+    "rk3_f64_16_solid_np4": lambda: case_small("rk3_f64_16_solid_np4", 64, omega_z=-1.4,
+                                               boundaries="solid", nprocs=4),
     "euler_f64_16": case_euler,
     "diag_f64_16": lambda: case_diag("diag_f64_16", 64),
     "diag_f32_16": lambda: case_diag("diag_f32_16", 32),

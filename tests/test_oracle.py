@@ -18,9 +18,11 @@ SMALL = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, 
 def _stepper_from_fixture(f):
     prec = int(f["prec"])
     box = tuple(float(b) for b in f["box"])
+    bc = str(f["boundaries"]) if "boundaries" in f else "zero"
+    pz = bool(f["periodicz"]) if "periodicz" in f else False
     p = oracle.params(f["psi0"].shape, box, tuple(float(h) for h in f["harm_w"]),
                       omega_z=float(f["omega_z"]), dt=complex(f["dt"]),
-                      gamma=float(f["gamma"]))
+                      gamma=float(f["gamma"]), boundaries=bc, periodicz=pz)
     return oracle.Stepper(p, f["psi0"], prec=prec), prec
 
 
