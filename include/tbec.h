@@ -169,6 +169,15 @@ int tbec_set_norm(tbec_handle *h, double norm);
  * segment (256 down to 8 planes) that still gives >= 1024 workgroups, which is also
  * the default at creation.  Drops captured graphs. */
 int tbec_set_tuning(tbec_handle *h, int tile, int seg);
+/* Boundary faces, as ghost_comm fills them (src/ghost.f90:27-192; `boundaries` and
+ * `periodicz` of src/cparam.local): TBEC_BC_ZERO (default: ghosts 0), TBEC_BC_SOLID
+ * (mirror about the edge point: ghost 0 = psi 2, -1 = 3, -2 = 4), TBEC_BC_PERIODIC
+ * (x, y and z periodic, as the reference's periodic process topology).  periodicz = 1
+ * makes z periodic whatever the boundaries.  Solid and periodic faces are refilled
+ * before every stage (a pass over the 6 faces).  Drops captured graphs. */
+enum { TBEC_BC_ZERO = 0, TBEC_BC_SOLID = 1, TBEC_BC_PERIODIC = 2 };
+int tbec_set_boundaries(tbec_handle *h, int boundaries, int periodicz);
+
 /* How the low-storage RK3 carries w between stages (src/run.f90:183-191).
  * stored = 1: the reference's scheme, w in its own array (10 HBM transfers of a complex
  * value per point and step).  stored = 0 (default): w is rebuilt pointwise from the two

@@ -146,3 +146,32 @@ def test_driver_writes_velocity_and_qpressure(tmp_path):
         assert list(np.frombuffer(rec[:8], np.int32)) == [0, 0]
         got = np.frombuffer(rec[8:], np.float64).reshape(3, 16, 16, 16)
         assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), name
+
+
+@pytest.mark.gpu
+def test_driver_solid_boundaries_match_reference(tmp_path):
+    """boundaries='solid' through the driver's namelist: 20 steps from the solid-boundary
+    reference fixture's psi0 equal the reference's output (FP64, 1e-12)."""
+    _need_driver()
+    f = np.load(os.path.join(ROOT, "tests", "golden", "rk3_f64_16_solid.npz"))
+    d = tmp_path / "data" / "proc000"
+    d.mkdir(parents=True)
+    # restart file: [itime] [t] [Psi] (src/initial.f90:796-818)
+    with open(d / "var.dat", "wb") as fh:
+        for payload in (np.array([1], np.int32).tobytes(), np.array([0.0]).tobytes(),
+                        np.ascontiguousarray(f["psi0"]).tobytes()):
+            n = np.array([len(payload)], np.int32).tobytes()
+            fh.write(n + payload + n)
+    box = tuple(float(b) for b in f["box"])
+    h = tuple(float(x) for x in f["harm_w"])
+    nml = tmp_path / "run.nml"
+    _nml(nml, Nx=16, Ny=16, Nz=16, Lx=box[0], Ly=box[1], Lz=box[2], harm_wx=h[0],
+         harm_wy=h[1], harm_wz=h[2], omega_z=float(f["omega_z"]), init_cond="restart",
+         boundaries="solid", nsteps=20, shots=20, out_dir=str(tmp_path / "data"))
+    print(f"\n[case] driver solid boundaries 16^3", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, psi = _read_var(d / "var0001.dat", (16, 16, 16))
+    ref = f["psi"]
+    assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()

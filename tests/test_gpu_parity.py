@@ -30,7 +30,8 @@ def announce(msg):
 @pytest.mark.parametrize("name", SMALL)
 def test_rk3_vs_reference_fixture(name, stored):
     """Both ways of carrying w (rebuilt from the psi pair, stored) against the
-    reference's own output."""
+    reference's own output, for every fixture: rotation, dissipation, complex dt, an
+    anisotropic grid, 4 ranks, and zero / solid / periodic / z-periodic boundaries."""
     f = np.load(os.path.join(GOLDEN, name + ".npz"))
     prec = int(f["prec"])
     announce(f"{name} shape={f['psi0'].shape} prec={prec} w_stored={stored}")
@@ -39,6 +40,8 @@ def test_rk3_vs_reference_fixture(name, stored):
                          omega_z=float(f["omega_z"]), dt=complex(f["dt"]),
                          gamma=float(f["gamma"]), prec=prec) as c:
         c.set_w_storage(stored)
+        if "boundaries" in f:
+            c.set_boundaries(str(f["boundaries"]), bool(f["periodicz"]))
         c.upload(f["psi0"])
         c.step(int(f["steps"]))
         got = c.download()

@@ -95,6 +95,72 @@ hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool ro
   }
 }
 
+// ------------------------------------------------------------- ghost fill ----
+// ghost_comm's boundary faces (src/ghost.f90:27-192) on the padded array: one thread per
+// face row (two axes' interior point), writing the 3 ghosts on both sides of the third
+// axis.  bc: 0 zero, 1 solid (ghost -g = psi g, 0-based), 2 periodic (ghost -g =
+// psi n-g); z follows bc unless zperiodic.  Corners are not touched (they stay zero);
+// the stencils never read them.
+template <typename T>
+__global__ void k_fill_ghosts(cplx<T> *__restrict__ psi, int nx, int ny, int nz, long long sy,
+                              long long sx, long long base, int bc, int zperiodic) {
+  const long long fx = (long long)ny * nz, fy = (long long)nx * nz, fz = (long long)nx * ny;
+  const long long total = fx + fy + fz;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int a, n, b = bc;
+    long long row, s;
+    if (t < fx) {
+      const long long j = t / nz, k = t % nz;
+      a = 0; n = nx; s = sx; row = base + j * sy + k;
+    } else if (t < fx + fy) {
+      const long long q = t - fx, i = q / nz, k = q % nz;
+      a = 1; n = ny; s = sy; row = base + i * sx + k;
+    } else {
+      const long long q = t - fx - fy, i = q / ny, j = q % ny;
+      a = 2; n = nz; s = 1; row = base + i * sx + j * sy;
+      if (zperiodic) b = 2;
+    }
+    (void)a;
+    for (int g = 1; g <= 3; ++g) {
+      cplx<T> lo{T(0), T(0)}, hi{T(0), T(0)};
+      if (b == 1) {
+        lo = psi[row + (long long)g * s];
+        hi = psi[row + (long long)(n - 1 - g) * s];
+      } else if (b == 2) {
+        lo = psi[row + (long long)(n - g) * s];
+        hi = psi[row + (long long)(g - 1) * s];
+      }
+      psi[row - (long long)g * s] = lo;
+      psi[row + (long long)(n - 1 + g) * s] = hi;
+    }
+  }
+}
+
+hipError_t fill_ghosts(tbec_handle *h, void *buf, int bc, int zperiodic) {
+  const long long total = (long long)h->p.ny * h->p.nz + (long long)h->p.nx * h->p.nz +
+                          (long long)h->p.nx * h->p.ny;
+  const int grid = (int)std::max<long long>(1, std::min<long long>((total + 255) / 256, 4096));
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_fill_ghosts<float>, dim3(grid), dim3(256), 0, h->stream,
+                       (cplx<float> *)buf, h->p.nx, h->p.ny, h->p.nz, h->sy, h->sx, h->base,
+                       bc, zperiodic);
+  else
+    hipLaunchKernelGGL(k_fill_ghosts<double>, dim3(grid), dim3(256), 0, h->stream,
+                       (cplx<double> *)buf, h->p.nx, h->p.ny, h->p.nz, h->sy, h->sx, h->base,
+                       bc, zperiodic);
+  return hipGetLastError();
+}
+
+// Fill the ghosts of the current psi when the boundaries are not all zero (the zero
+// halo is permanent and needs no pass).
+int refresh_ghosts(tbec_handle *h) {
+  if (h->bc == TBEC_BC_ZERO && !h->periodicz) return 0;
+  hipError_t e = fill_ghosts(h, h->psi[h->cur], h->bc, h->periodicz);
+  if (e != hipSuccess) return fail(TBEC_ERR_HIP, "ghost fill failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
 // ------------------------------------------------------------- small kernels ----
 template <typename T>
 __global__ void k_unpack(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst, int nx,
@@ -339,6 +405,8 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     a.w = a.psi_out;
     a.wrec = kind == ST_RK2 ? 3.0 : 1.0;
   }
+  int rcg = refresh_ghosts(h);
+  if (rcg) return rcg;
   hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, handle_rot(h), want_red,
                               h->stream);
   if (e != hipSuccess)
@@ -593,6 +661,21 @@ int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
   tbec_drop_graphs(h);
   h->tile = tile;
   h->seg = seg > 0 ? seg : auto_seg(tiles_per_plane(h->p, tile), h->p.nx, kAutoSegBlocks);
+  return 0;
+}
+
+int tbec_set_boundaries(tbec_handle *h, int boundaries, int periodicz) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (boundaries < TBEC_BC_ZERO || boundaries > TBEC_BC_PERIODIC)
+    return fail(TBEC_ERR_ARG, "boundaries must be TBEC_BC_ZERO, _SOLID or _PERIODIC");
+  if (periodicz != 0 && periodicz != 1) return fail(TBEC_ERR_ARG, "periodicz must be 0 or 1");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  tbec_drop_graphs(h);
+  h->bc = boundaries;
+  h->periodicz = periodicz;
+  // zero faces leave the permanent zero halo the zero boundary relies on
+  for (int q = 0; q < 2; ++q) HIPCHK(fill_ghosts(h, h->psi[q], TBEC_BC_ZERO, 0));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

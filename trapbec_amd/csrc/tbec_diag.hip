@@ -5,10 +5,10 @@
 //
 // The handle's w array (not carried across steps) is the scratch: its two real halves
 // hold the phase or the density, and the Laplacian of the density, over the padded
-// grid.  The padded psi halo is zero, so the ghost phase is atan2(0, 0) = 0 and the ghost
-// density 0, as after the reference's zero-boundary ghost fill; corners are zero too,
-// which is the reference's one-rank case.  Sums keep the reference's operand order and
-// are not contracted into FMAs.
+// grid.  The ghost faces are filled first as ghost_comm fills them (zero, solid or
+// periodic; with zero boundaries the ghost phase is atan2(0, 0) = 0 and the ghost density
+// 0); corners stay zero, which is the reference's one-rank case.  Sums keep the
+// reference's operand order and are not contracted into FMAs.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -145,6 +145,7 @@ int run_field(tbec_handle *h, bool velocity, void *out_host) {
 int field(tbec_handle *h, bool velocity, void *out_host) {
   if (!h || !out_host) return set_error(TBEC_ERR_ARG, "null argument");
   int rc = tbec_materialize_scale(h);   // a pending imaginary-time scale
+  if (!rc) rc = refresh_ghosts(h);      // ghost_comm before get_phase / get_qpressure
   if (rc) return rc;
   return h->p.prec == TBEC_F32 ? run_field<float>(h, velocity, out_host)
                                : run_field<double>(h, velocity, out_host);

@@ -62,6 +62,7 @@ struct tbec_handle {
   int tile = 0, seg = 64;
   int remap = 1;                        // XCD-aware tile order
   int w_stored = 0;                     // 1: RK3 stores w (10 transfers), 0: rebuilt (8)
+  int bc = 0, periodicz = 0;            // boundary faces (TBEC_BC_*), z periodic
   int csize = 8;
   double hx = 0, hy = 0, hz = 0;
   double dt_re = 0, dt_im = 0;          // dt*(1 - i*gamma)
@@ -91,5 +92,8 @@ int set_error(int code, const std::string &msg);
 
 int tbec_reset_state(tbec_handle *h);
 int tbec_materialize_scale(tbec_handle *h);
+namespace tbec {
+int refresh_ghosts(tbec_handle *h);   // ghost faces of the current psi (non-zero boundaries)
+}  // namespace tbec
 void tbec_drop_graphs(tbec_handle *h);
 int fill_args_count(tbec_handle *h);

@@ -11,6 +11,7 @@ module tbec_iface
   integer(c_int), parameter :: TBEC_F32 = 32, TBEC_F64 = 64
   integer(c_int), parameter :: TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1, &
                                 TBEC_SUM_CENTRED = 2
+  integer(c_int), parameter :: TBEC_BC_ZERO = 0, TBEC_BC_SOLID = 1, TBEC_BC_PERIODIC = 2
   integer(c_int), parameter :: TBEC_REAL_RK3 = 0, TBEC_IMAG_RK3 = 1, TBEC_IMAG_EULER_REF = 2
 
   type, bind(C) :: tbec_params
@@ -52,6 +53,12 @@ module tbec_iface
     integer(c_int) function tbec_download_psi(h, psi) bind(C, name="tbec_download_psi")
       import :: c_int, c_ptr
       type(c_ptr), value :: h, psi
+    end function
+    integer(c_int) function tbec_set_boundaries(h, bc, periodicz) &
+        bind(C, name="tbec_set_boundaries")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: bc, periodicz
     end function
     integer(c_int) function tbec_velocity(h, vel) bind(C, name="tbec_velocity")
       import :: c_int, c_ptr

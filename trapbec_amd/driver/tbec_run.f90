@@ -36,7 +36,8 @@ program tbec_run
   integer(int64) :: seed = 20261203_int64
   real(dp) :: noise_amp = 0.0_dp, phase_noise_amp = 0.0_dp
   logical :: imaginary_time = .false., ic_only = .false., full_binary_output = .true.
-  logical :: vel_print = .false., qpress_print = .false.
+  logical :: vel_print = .false., qpress_print = .false., periodicz = .false.
+  character(len=16) :: boundaries = 'zero'
   integer :: imag_nsteps = 0, imag_check = 50
   real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
   character(len=256) :: out_dir = 'data'
@@ -45,7 +46,7 @@ program tbec_run
     trap_omega, mu, g_phys, tf_norm, psi_scale, n_lines, line_x, line_y, line_q, xi, &
     ring_radius, seed, noise_amp, phase_noise_amp, imaginary_time, ic_only, &
     full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir, &
-    vel_print, qpress_print
+    vel_print, qpress_print, boundaries, periodicz
 
   character(len=512) :: nml_file
   integer :: u, ios, itime, nstart, l
@@ -125,6 +126,7 @@ program tbec_run
   p%omega_z = omega_z; p%c0 = c0; p%gnl = gnl
   p%dt_re = dt_re; p%dt_im = dt_im; p%gamma = gamma
   call check(tbec_create(p, h))
+  call set_bc(h)
   call upload()
 
   if (imaginary_time .and. nstart == 1) then
@@ -200,6 +202,7 @@ contains
     q%dt_im = -abs(dt_im)
     if (dt_im == 0.0_dp) q%dt_im = -abs(dt_re)
     call check(tbec_create(q, hi))
+    call set_bc(hi)
     if (prec == 64) then
       call check(tbec_upload_psi(hi, c_loc(psi)))
     else
@@ -250,6 +253,25 @@ contains
       write(uv) cmplx(psi, kind=real32)
     end if
     close(uv)
+  end subroutine
+
+  ! boundaries / periodicz as in src/cparam.local, applied by ghost_comm's rules
+  subroutine set_bc(hh)
+    type(c_ptr), intent(in) :: hh
+    integer(c_int) :: bc, pz
+    select case (trim(boundaries))
+    case ('zero')
+      bc = TBEC_BC_ZERO
+    case ('solid')
+      bc = TBEC_BC_SOLID
+    case ('periodic')
+      bc = TBEC_BC_PERIODIC
+    case default
+      call emergency_stop('wrong boundaries set')
+    end select
+    pz = 0
+    if (periodicz) pz = 1
+    call check(tbec_set_boundaries(hh, bc, pz))
   end subroutine
 
   ! vel%02d.dat / qpress%02d.dat: one record coords(2), field(nz, ny, nx, 3) in the run's

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "libtbec.so")
 TBEC_F32, TBEC_F64 = 32, 64
 SUM_REFERENCE, SUM_COMPENSATED, SUM_CENTRED = 0, 1, 2
 NUM_TILES = 5                   # tile shape ids 0..4 per precision
+BC_ZERO, BC_SOLID, BC_PERIODIC = 0, 1, 2
+BOUNDARIES = {"zero": BC_ZERO, "solid": BC_SOLID, "periodic": BC_PERIODIC}
 REAL_RK3, IMAG_RK3, IMAG_EULER_REF = 0, 1, 2
 ERRORS = {-1: "TBEC_ERR_ARG", -2: "TBEC_ERR_OOM", -3: "TBEC_ERR_HIP",
           -4: "TBEC_ERR_NONFINITE", -5: "TBEC_ERR_STATE"}
@@ -71,7 +73,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
            "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
-           "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure"]
+           "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries"]
 
 _lib = None
 
@@ -98,6 +100,8 @@ def load(path: str = LIB_PATH):
     L.tbec_set_norm.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
+    L.tbec_set_w_storage.argtypes = [H, C.c_int]
+    L.tbec_set_boundaries.argtypes = [H, C.c_int, C.c_int]
     L.tbec_init_state.argtypes = [H, C.POINTER(TbecIc)]
     L.tbec_sync.argtypes = [H]
     L.tbec_stream.argtypes = [H]
@@ -257,6 +261,12 @@ class Condensate:
 
     def set_xcd_remap(self, on: bool):
         _check(load().tbec_set_xcd_remap(self._h, int(bool(on))))
+
+    def set_boundaries(self, boundaries="zero", periodicz: bool = False):
+        """Boundary faces as the reference's ghost_comm fills them: "zero", "solid"
+        (mirror about the edge point) or "periodic"; periodicz makes z periodic."""
+        bc = BOUNDARIES[boundaries] if isinstance(boundaries, str) else int(boundaries)
+        _check(load().tbec_set_boundaries(self._h, bc, int(bool(periodicz))))
 
     def set_w_storage(self, stored: bool):
         """True: RK3 keeps w in its own array (the reference's scheme, 10 transfers per
