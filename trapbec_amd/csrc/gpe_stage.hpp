@@ -151,14 +151,14 @@ template <> struct VecIO<double, 1> {
 
 // Raw buffer access for the stage kernel: a uniform 64-bit plane base in a buffer
 // resource (SGPRs) plus a 32-bit per-lane byte offset (one VGPR), instead of a 64-bit
-// address per lane.  The resource's record count is the number of bytes left in the
-// allocation from that base, so every access is range-checked against it.
+// address per lane.  The resource's record count is one x-plane, so every access is
+// range-checked against the plane it belongs to; the march only touches planes
+// -3 .. nx+3, which the padded layout holds.
 typedef unsigned int tbec_u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kRsrcWord3 = 0x00020000;   // gfx9-family raw buffer, no swizzle
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, long long bytes) {
-  const int n = (int)(bytes > 0x7fffffffLL ? 0x7fffffffLL : (bytes < 0 ? 0 : bytes));
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, n, kRsrcWord3);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, kRsrcWord3);
 }
 
 typedef unsigned int tbec_u32x2 __attribute__((ext_vector_type(2)));
@@ -340,17 +340,18 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // when the queue advances) and the LDS buffer / halo register parity is static too.
   C q[8][VZ];
   C hb[2][VZ];
-  // buffer resources for x-plane `x` of the three arrays (bytes left in the allocation)
-  const long long left0 = (a.elems - rb) * (long long)sizeof(C);
-  auto rs_in = [&](int x) {
-    return plane_rsrc(pin + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
-  };
-  auto rs_w = [&](int x) {
-    return plane_rsrc(pw + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
-  };
-  auto rs_out = [&](int x) {
-    return plane_rsrc(pout + (long long)x * sx, left0 - (long long)x * sx * (long long)sizeof(C));
-  };
+  // buffer resources for x-plane `x` of the three arrays: base = the plane's row -3,
+  // column -4; record count = one plane (sx complex values, < 2^31 bytes: checked by
+  // the host)
+  // Each stream keeps a pointer to the plane it touches next and advances it by one
+  // plane (sx) per plane: a 64-bit scalar add, no multiply by the loop-varying x.
+  const int pbytes = (int)(sx * (long long)sizeof(C));
+  auto rs = [&](const C *plane) { return plane_rsrc(plane, pbytes); };
+  const C *p_own = pin + (long long)(xs + 4) * sx;     // own points of plane x+4
+  const C *p_halo = pin + (long long)(xs + 1) * sx;    // halo of plane x+1
+  C *p_wnext = pw + (long long)(xs + 1) * sx;          // w (or psi_{s-1}) of plane x+1
+  C *p_out = pout + (long long)xs * sx;                // outputs of plane x
+  C *p_w = pw + (long long)xs * sx;
   const uint32_t ownb = own * (uint32_t)sizeof(C), hoffb = hoff * (uint32_t)sizeof(C);
   auto ldv = [&](__amdgpu_buffer_rsrc_t r, uint32_t off, C *o) {
     BufIO<T, VZ>::ld(r, off, o);
@@ -360,11 +361,11 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     }
   };
 #pragma unroll
-  for (int k = 0; k < 7; ++k) ldv(rs_in(xs - 3 + k), ownb, q[k]);
-  if (hl) ldv(rs_in(xs), hoffb, hb[0]);
+  for (int k = 0; k < 7; ++k) ldv(rs(pin + (long long)(xs - 3 + k) * sx), ownb, q[k]);
+  if (hl) ldv(rs(pin + (long long)xs * sx), hoffb, hb[0]);
   // w of the first plane (stages that read w); later planes are prefetched one ahead
   C wq[2][VZ];
-  if (stage_reads_w(a.stage)) BufIO<T, VZ>::ld(rs_w(xs), ownb, wq[0]);
+  if (stage_reads_w(a.stage)) BufIO<T, VZ>::ld(rs(p_w), ownb, wq[0]);
 
   const T k2x = T(a.k2[0]), k2y = T(a.k2[1]), k2z = T(a.k2[2]);
   const T k1x = T(a.k1[0]), k1y = T(a.k1[1]);
@@ -396,9 +397,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     constexpr int R = decltype(Rc)::value;   // (x - xs) & 7
     constexpr int buf = R & 1;
     // prefetch plane x+4 (own) and the halo of plane x+1
-    ldv(rs_in(x + 4), ownb, q[(R + 7) & 7]);
-    if (hl) ldv(rs_in(x + 1), hoffb, hb[buf ^ 1]);
-    if (stage_reads_w(stage) && x + 1 < xe) BufIO<T, VZ>::ld(rs_w(x + 1), ownb, wq[buf ^ 1]);
+    ldv(rs(p_own), ownb, q[(R + 7) & 7]);
+    if (hl) ldv(rs(p_halo), hoffb, hb[buf ^ 1]);
+    if (stage_reads_w(stage) && x + 1 < xe) BufIO<T, VZ>::ld(rs(p_wnext), ownb, wq[buf ^ 1]);
     const C *wv = wq[buf];
 
     // stage centre plane + halo into LDS
@@ -493,14 +494,14 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     }
     // stores (interior points only; padding stays zero)
     if (yin) {
-      C *po = pout + (long long)x * sx;   // uniform plane bases
-      C *pwx = pw + (long long)x * sx;
+      C *po = p_out;   // uniform plane bases
+      C *pwx = p_w;
       if (full) {
         if (stage == ST_RHS) {
-          BufIO<T, VZ>::st(rs_w(x), ownb, wo);
+          BufIO<T, VZ>::st(rs(p_w), ownb, wo);
         } else {
-          BufIO<T, VZ>::st(rs_out(x), ownb, out);
-          if (store_w) BufIO<T, VZ>::st(rs_w(x), ownb, wo);
+          BufIO<T, VZ>::st(rs(p_out), ownb, out);
+          if (store_w) BufIO<T, VZ>::st(rs(p_w), ownb, wo);
         }
       } else {
 #pragma unroll
@@ -515,6 +516,11 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         }
       }
     }
+    p_own += sx;
+    p_halo += sx;
+    p_wnext += sx;
+    p_out += sx;
+    p_w += sx;
   };
 
   using I0 = std::integral_constant<int, 0>;

@@ -463,6 +463,10 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   h->seg = auto_seg(tiles_per_plane(*p, h->tile), p->nx, kAutoSegBlocks);
   Layout L;
   compute_layout(*p, L);
+  if (L.sx * L.csize > 0x7fffffffLL) {   // one x-plane is one buffer record range
+    delete h;
+    return fail(TBEC_ERR_ARG, "x-plane of %lld bytes exceeds 2^31", L.sx * L.csize);
+  }
   h->csize = L.csize;
   h->hx = L.hx;
   h->hy = L.hy;

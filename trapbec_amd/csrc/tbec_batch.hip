@@ -297,6 +297,10 @@ int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out) {
   for (int i = 0; i < n; ++i) {
     compute_layout(b->p[i], b->L[i]);
     b->rot = b->rot || rot_enabled(b->p[i]);
+    if (b->L[i].sx * b->L[i].csize > 0x7fffffffLL) {
+      delete b;
+      return set_error(TBEC_ERR_ARG, "an instance's x-plane exceeds 2^31 bytes");
+    }
   }
   auto cleanup = [&](int rc) { tbec_batch_destroy(b); return rc; };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
