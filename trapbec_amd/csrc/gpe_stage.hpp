@@ -543,11 +543,13 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   }
 }
 
-// Minimum waves per SIMD asked of the compiler.  512-thread FP32 real-time kernels are
-// held to 128 VGPRs so that two workgroups share a CU (the rotating variant needs 134
-// otherwise, which halves the resident waves); elsewhere the floor would cost spills.
+// Minimum waves per SIMD asked of the compiler.  512-thread FP32 kernels are held to
+// 128 VGPRs so that two workgroups share a CU: the rotating and the imaginary-time
+// variants need 130-145 otherwise, which halves the resident waves (measured: 20-30%
+// slower even where the floor makes the compiler spill a few registers).  FP64 kernels
+// would spill heavily and keep the compiler's choice.
 template <typename T, int NT, bool IMAG> constexpr int stage_min_waves() {
-  return (sizeof(T) == 4 && NT == 512 && !IMAG) ? 4 : 1;
+  return (sizeof(T) == 4 && NT == 512) ? 4 : 1;
 }
 
 template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
