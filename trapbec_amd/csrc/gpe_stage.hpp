@@ -597,7 +597,7 @@ struct BatchLaunch {
 };
 
 template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
-__global__ void __launch_bounds__((TZ / VZ) * TY)
+__global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage_batch(const BatchLaunch L) {
   constexpr int NT = (TZ / VZ) * TY;
   for (int item = blockIdx.x; item < L.total_items; item += gridDim.x) {
