@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--fine-dtau", type=float, default=8e-4)
     ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--test", type=float, nargs=5, default=[1.0, 0.8, 3000.0, 0.5, 0.0],
+                    help="held-out parameters: w_perp w_z g Omega log_N")
     ap.add_argument("--out", default="gpurun_out/warmstart")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
@@ -185,7 +187,7 @@ def main():
         val_pred = m.forward(X[n_tr:]) / scale
         val_mse = float(np.mean((val_pred - Y[n_tr:]) ** 2))
         base_mse = float(np.mean((Y[:n_tr].mean(axis=0) - Y[n_tr:]) ** 2))
-        test = np.array([1.0, 0.8, 3000.0, 0.5, 0.0])
+        test = np.array(a.test)
         coarse = (m.forward(features(test[None]))[0] / scale).reshape(3, COARSE, COARSE, COARSE)
     # the network predicts the state at atom number N; the refinement runs at norm 1
     coarse[0] /= math.sqrt(math.exp(test[4]))
@@ -202,6 +204,12 @@ def main():
     r_ml = refine_until(pr, e_t, r_t, a.max_steps, coarse=coarse, tuning=tuning)
     print(f"[warmstart] MLP start reaches E <= {e_t:.10f}, residual <= {r_t:.3e}: "
           f"{r_ml['steps']} steps {r_ml['wall_s']:.1f} s", flush=True)
+    r_ml_tol = None
+    if r_tf["converged"]:
+        # both to the stop rule: the north-star comparison when the TF start gets there
+        r_ml_tol = refine(pr, None, a.tol, a.max_steps, coarse=coarse, tuning=tuning)
+        print(f"[warmstart] MLP start to tol {a.tol:g}: {r_ml_tol['steps']} steps "
+              f"{r_ml_tol['wall_s']:.1f} s, E {r_ml_tol['final_energy']:.10f}", flush=True)
     out = {"samples": a.samples, "gen_s": t_gen, "train_s": t_train,
            "train_loss_first_last": [float(hist[0]), float(hist[-1])],
            "val_mse": val_mse, "val_mse_mean_predictor": base_mse,
@@ -210,8 +218,12 @@ def main():
            "mlp_start": r_ml,
            "criterion": "steps until E and GP residual reach the TF start's final values",
            "speedup_steps": r_tf["steps"] / max(1, r_ml["steps"]),
-           "speedup_wall": r_tf["wall_s"] / max(1e-9, r_ml["wall_s"])}
-    with open(os.path.join(a.out, "warmstart.json"), "w") as f:
+           "speedup_wall": r_tf["wall_s"] / max(1e-9, r_ml["wall_s"]),
+           "mlp_start_to_tol": r_ml_tol,
+           "speedup_steps_to_tol": (r_tf["steps"] / max(1, r_ml_tol["steps"])
+                                    if r_ml_tol and r_ml_tol["converged"] else None)}
+    tag = "_".join(f"{v:g}" for v in a.test)
+    with open(os.path.join(a.out, f"warmstart_{tag}.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
 
