@@ -109,7 +109,9 @@ int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout);
  * buffer (interior layout).  Test hook. */
 int tbec_rhs(tbec_handle *h, void *rhs_host);
 
-/* Diagnostics of the current psi; synchronises. */
+/* Diagnostics of the current psi; synchronises.  norm, energy and mu come from one
+ * rhs pass; the GP residual ||(H - mu) psi|| / ||psi|| from a second pass with c0 = 2 mu,
+ * which makes it the norm of that pass's rhs (no cancellation near a ground state). */
 int tbec_diagnose(tbec_handle *h, tbec_diag *out);
 
 /* Imaginary-time ground state: TBEC_IMAG_RK3 steps with dtau = -dt_im until

@@ -252,6 +252,23 @@ k_fin_energy(const double *part, int nb, double *st, int *flag, int check_conv) 
   if (threadIdx.x == 0) fin_energy(s, st, flag, check_conv);
 }
 
+// Diagnostics: c0 = 2*mu for the residual pass.  With rhs(c) = K(psi) + c*psi and
+// H psi = -K/2, (H - mu) psi = -rhs(2 mu)/2, so the pass's sum of |rhs|^2 is the
+// residual without the cancellation of (1/4) R - a S1 + a^2 N at a = c0/2 - mu.
+__global__ void k_set_resid_c0(double *st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st[ST_DC0] = 2.0 * st[ST_MU];
+}
+
+__global__ void __launch_bounds__(256)
+k_fin_resid(const double *part, int nb, double *st) {
+  double s[4];
+  sum_partials(part, 0, nb, s, false);
+  if (threadIdx.x == 0) {
+    const double R = s[3] * st[ST_DV], N = st[ST_N];
+    st[ST_RES] = N > 0.0 ? sqrt(0.25 * R / N) : 0.0;
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_fin_norm(const double *part, int nb, double *st, int *flag) {
   if (flag[0]) return;
@@ -396,6 +413,7 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     if (kind == ST_RK2) a.prev_scale = h->dstate + ST_SCALE;
   } else if (reduce) {
     a.scale = h->dstate + ST_SCALE;
+    if (h->resid_pass) a.c0dev = h->dstate + ST_DC0;
   }
   if (h->w_stored) {
     a.store_w = kind == ST_RK1 || kind == ST_RK2;
@@ -624,6 +642,15 @@ int tbec_diagnose(tbec_handle *h, tbec_diag *out) {
   const int nb_rhs = fill_args_count(h);
   hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,
                      h->dstate, h->dflag, 0);
+  HIPCHK(hipGetLastError());
+  // second pass for the GP residual, with c0 = 2*mu of the first
+  hipLaunchKernelGGL(k_set_resid_c0, dim3(1), dim3(64), 0, h->stream, h->dstate);
+  h->resid_pass = true;
+  rc = enqueue_stage(h, ST_RHS, false, true);
+  h->resid_pass = false;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,
+                     h->dstate);
   HIPCHK(hipGetLastError());
   const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
   const int nbi = grid_for(n);

@@ -25,6 +25,7 @@ enum StateSlot {
   ST_N, ST_E, ST_MU, ST_RES, ST_REL,
   ST_MAXA, ST_MINA, ST_SUMA, ST_SUMA2,
   ST_HITS,        // consecutive steps below the tolerance
+  ST_DC0,         // diagnostics: 2*mu, the c0 of the residual pass
   ST_COUNT
 };
 
@@ -63,6 +64,7 @@ struct tbec_handle {
   int remap = 1;                        // XCD-aware tile order
   int w_stored = 0;                     // 1: RK3 stores w (10 transfers), 0: rebuilt (8)
   int bc = 0, periodicz = 0;            // boundary faces (TBEC_BC_*), z periodic
+  bool resid_pass = false;              // next ST_RHS reduction runs with c0 = ST_DC0
   int csize = 8;
   double hx = 0, hy = 0, hz = 0;
   double dt_re = 0, dt_im = 0;          // dt*(1 - i*gamma)
