@@ -224,6 +224,30 @@ def test_imag_solve_converges_harmonic():
     assert d["residual"] < 1e-3
 
 
+def test_diagnose_residual_near_ground_state():
+    """Near a ground state the GP residual is a small difference of large terms; the
+    second pass with c0 = 2 mu keeps it resolved in FP32 (a single-pass formula reads 0
+    there).  FP64 ground state of the isotropic harmonic trap, then the same state in an
+    FP32 handle."""
+    shape, box = (32, 32, 32), (12.0, 12.0, 12.0)
+    announce("diagnose residual near a ground state")
+    pr = initial.Problem(shape, box, (1.0, 1.0, 1.0), 20.0, 0.0, -1e-2j)
+    X, Y, Z = initial.mesh(shape, box)
+    psi0 = np.exp(-0.4 * (X ** 2 + Y ** 2 + Z ** 2)).astype(complex)
+    with tbec.Condensate.from_problem(pr, prec=64) as c:
+        c.upload(psi0)
+        c.imag_solve(20000, 1e-13, 1.0, 100)
+        d64 = c.diagnose()
+        gs = c.download()
+    with tbec.Condensate.from_problem(pr, prec=32) as c:
+        c.upload(gs)
+        d32 = c.diagnose()
+    print(f"residual fp64 {d64['residual']:.3e} fp32 {d32['residual']:.3e}", flush=True)
+    assert d64["residual"] < 1e-6
+    assert 0.0 < d32["residual"] < 1e-4
+    assert d32["energy"] == pytest.approx(d64["energy"], rel=1e-6)
+
+
 def test_fp32_compensated_vs_fp64():
     """FP32 with compensated stencil sums (TwoSum and centred-difference forms) tracks
     FP64 on a 128^3 slice of the throughput workload over 200 steps (relative L2 < 1e-5)
