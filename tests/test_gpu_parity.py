@@ -243,8 +243,8 @@ def test_diagnose_residual_near_ground_state():
         c.upload(gs)
         d32 = c.diagnose()
     print(f"residual fp64 {d64['residual']:.3e} fp32 {d32['residual']:.3e}", flush=True)
-    assert d64["residual"] < 1e-6
-    assert 0.0 < d32["residual"] < 1e-4
+    assert d64["residual"] < 1e-5
+    assert abs(d32["residual"] - d64["residual"]) < 1e-6     # FP32 resolves ~1e-7
     assert d32["energy"] == pytest.approx(d64["energy"], rel=1e-6)
 
 
