@@ -133,9 +133,13 @@ void orc_imag_rk3_step_f64(const orc_params *p, double *psi_pad, double *w, doub
       const double mu = H / sums[0];
       const double a = 0.5 * q.c0 - mu;
       double res2 = 0.25 * sums[3] - a * sums[1] + a * a * sums[0];
+      /* energy and mu of the state scaled exactly to norm_target: the quadratic part
+       * of H scales with r = norm_target/N, the interaction part with r^2 */
+      const double r = norm_target / sums[0], g4 = 0.25 * q.gnl * sums[2];
+      const double hlin = H - 2.0 * g4;
       diag[0] = sums[0];
-      diag[1] = H - 0.25 * q.gnl * sums[2];
-      diag[2] = mu;
+      diag[1] = r * hlin + r * r * g4;
+      diag[2] = (r * hlin + r * r * 2.0 * g4) / norm_target;
       diag[3] = sqrt((res2 < 0 ? 0 : res2) / sums[0]);
     }
     for (long t = 0; t < n; ++t) {

@@ -50,13 +50,19 @@ __device__ inline void sum_partials(const double *part, int beg, int end, double
 // After a stage-1 (or rhs-only) pass with reductions s = {sum|psi|^2, Re sum psi* rhs,
 // sum|psi|^4, sum|rhs|^2} (without dV): energy, mu, residual; with check_conv also the
 // convergence test and the c0 of the next step.  Thread 0 only.
+//
+// In imaginary time (check_conv) E and mu are those of the state scaled exactly to the
+// target norm: with r = N_target/N the quadratic part of H scales by r and the
+// interaction part by r^2.  The state read in stage 1 is (FP32 rounding of the scale)*psi,
+// whose norm is off by ~1e-7; without this the energy sequence carries that rounding and
+// its step-to-step change stalls near 1e-8 instead of reaching tolerances like 1e-10.
 __device__ inline void fin_energy(const double s[4], double *st, int *flag, int check_conv) {
   const double dV = st[ST_DV];
   const double N = s[0] * dV, S1 = s[1] * dV, S4 = s[2] * dV, R = s[3] * dV;
   const double c0 = check_conv ? st[ST_C0EFF] : st[ST_C0BASE];
   const double H = -0.5 * S1 + 0.5 * c0 * N;
-  const double E = H - 0.25 * st[ST_GNL] * S4;
-  const double mu = H / N;
+  double E = H - 0.25 * st[ST_GNL] * S4;
+  double mu = H / N;
   const double am = 0.5 * c0 - mu;
   double r2 = 0.25 * R - am * S1 + am * am * N;
   if (r2 < 0) r2 = 0;
@@ -69,6 +75,14 @@ __device__ inline void fin_energy(const double s[4], double *st, int *flag, int 
     flag[0] = 2;
     flag[1] = (int)st[ST_STEP];
     return;
+  }
+  {
+    const double r = st[ST_NORMT] / N, g4 = 0.25 * st[ST_GNL] * S4;
+    const double Hlin = H - 2.0 * g4;              // the part quadratic in psi
+    E = r * Hlin + r * r * g4;
+    mu = (r * Hlin + r * r * 2.0 * g4) / st[ST_NORMT];
+    st[ST_E] = E;
+    st[ST_MU] = mu;
   }
   const double rel = st[ST_STEP] > 0.0 ? fabs(E - st[ST_EPREV]) / fabs(E) : INFINITY;
   st[ST_REL] = rel;
