@@ -256,7 +256,10 @@ k_fin_energy(const double *part, int nb, double *st, int *flag, int check_conv) 
 // H psi = -K/2, (H - mu) psi = -rhs(2 mu)/2, so the pass's sum of |rhs|^2 is the
 // residual without the cancellation of (1/4) R - a S1 + a^2 N at a = c0/2 - mu.
 __global__ void k_set_resid_c0(double *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st[ST_DC0] = 2.0 * st[ST_MU];
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double c = 2.0 * st[ST_MU];
+    st[ST_DC0] = st[ST_F32] != 0.0 ? (double)(float)c : c;   // as the kernel will read it
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -264,8 +267,12 @@ k_fin_resid(const double *part, int nb, double *st) {
   double s[4];
   sum_partials(part, 0, nb, s, false);
   if (threadIdx.x == 0) {
-    const double R = s[3] * st[ST_DV], N = st[ST_N];
-    st[ST_RES] = N > 0.0 ? sqrt(0.25 * R / N) : 0.0;
+    // (H - mu) psi = -rhs(c)/2 + a psi with a = c/2 - mu, |a| ~ rounding of c only
+    const double R = s[3] * st[ST_DV], S1 = s[1] * st[ST_DV], N = st[ST_N];
+    const double a = 0.5 * st[ST_DC0] - st[ST_MU];
+    double r2 = 0.25 * R - a * S1 + a * a * N;
+    if (r2 < 0) r2 = 0;
+    st[ST_RES] = N > 0.0 ? sqrt(r2 / N) : 0.0;
   }
 }
 
@@ -747,11 +754,14 @@ int tbec_reset_state(tbec_handle *h) {
   double *s = h->hstate;
   for (int i = 0; i < ST_COUNT; ++i) s[i] = 0.0;
   s[ST_SCALE] = 1.0;
-  s[ST_C0EFF] = s[ST_C0NEXT] = s[ST_C0BASE] = h->p.c0;
+  // c0 as the kernels of this precision read it
+  s[ST_C0EFF] = s[ST_C0NEXT] = s[ST_C0BASE] =
+      h->p.prec == TBEC_F32 ? (double)(float)h->p.c0 : h->p.c0;
   s[ST_GNL] = h->p.gnl;
   s[ST_DV] = h->hx * h->hy * h->hz;
   s[ST_NORMT] = h->norm_target;
   s[ST_TOL] = h->tol;
+  s[ST_F32] = h->p.prec == TBEC_F32 ? 1.0 : 0.0;
   HIPCHK(hipMemcpyAsync(h->dstate, s, sizeof(double) * ST_COUNT, hipMemcpyHostToDevice,
                         h->stream));
   for (int i = 0; i < 4; ++i) h->hflag[i] = 0;

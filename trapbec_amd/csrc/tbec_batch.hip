@@ -251,11 +251,13 @@ int reset_states(tbec_batch *b, double norm, double tol) {
     double *s = b->h_state + (size_t)i * ST_COUNT;
     for (int k = 0; k < ST_COUNT; ++k) s[k] = 0.0;
     s[ST_SCALE] = 1.0;
-    s[ST_C0EFF] = s[ST_C0NEXT] = s[ST_C0BASE] = b->p[i].c0;
+    s[ST_C0EFF] = s[ST_C0NEXT] = s[ST_C0BASE] =
+        b->prec == TBEC_F32 ? (double)(float)b->p[i].c0 : b->p[i].c0;
     s[ST_GNL] = b->p[i].gnl;
     s[ST_DV] = b->L[i].hx * b->L[i].hy * b->L[i].hz;
     s[ST_NORMT] = norm;
     s[ST_TOL] = tol;
+    s[ST_F32] = b->prec == TBEC_F32 ? 1.0 : 0.0;
     for (int k = 0; k < 4; ++k) b->h_flags[4 * i + k] = 0;
   }
   BCHK(hipMemcpyAsync(b->d_state, b->h_state, sizeof(double) * ST_COUNT * b->n,

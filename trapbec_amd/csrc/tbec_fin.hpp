@@ -87,7 +87,10 @@ __device__ inline void fin_energy(const double s[4], double *st, int *flag, int 
   const double rel = st[ST_STEP] > 0.0 ? fabs(E - st[ST_EPREV]) / fabs(E) : INFINITY;
   st[ST_REL] = rel;
   st[ST_EPREV] = E;
-  st[ST_C0NEXT] = st[ST_C0BASE] + 2.0 * mu;
+  // the kernels read c0 in their own precision: store it rounded so that the energy
+  // above (which adds c0*N/2 back) uses exactly the c0 the rhs was computed with
+  const double c0n = st[ST_C0BASE] + 2.0 * mu;
+  st[ST_C0NEXT] = st[ST_F32] != 0.0 ? (double)(float)c0n : c0n;
   st[ST_HITS] = rel < st[ST_TOL] ? st[ST_HITS] + 1.0 : 0.0;
   if (st[ST_HITS] >= (double)kConvergeHits) {
     flag[0] = 1;

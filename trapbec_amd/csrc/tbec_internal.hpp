@@ -26,6 +26,7 @@ enum StateSlot {
   ST_MAXA, ST_MINA, ST_SUMA, ST_SUMA2,
   ST_HITS,        // consecutive steps below the tolerance
   ST_DC0,         // diagnostics: 2*mu, the c0 of the residual pass
+  ST_F32,         // 1 when the kernels compute in FP32 (c0 values are stored rounded)
   ST_COUNT
 };
 
