@@ -153,6 +153,43 @@ def refine_until(pr, e_target, r_target, max_steps, chunk=500, coarse=None, star
                 "wall_s": time.time() - t0}
 
 
+def history(pr, max_steps, chunk=1000, coarse=None, start=None, prec=32, tuning=None):
+    """Plain renormalised imaginary time for max_steps (no stop rule) from a start,
+    recording (steps, E, GP residual) every `chunk` steps."""
+    out = []
+    with tbec.Condensate.from_problem(pr, prec=prec) as c:
+        if tuning is not None:
+            c.set_tuning(tuning[0], tuning[1])
+            c.set_xcd_remap(tuning[2])
+        if coarse is not None:
+            tbec.upload_coarse(c, coarse)
+        else:
+            c.upload(start)
+        d = c.diagnose()
+        out.append((0, d["energy"], d["residual"]))
+        t0 = last = time.time()
+        steps = 0
+        while steps < max_steps:
+            c.step(chunk, tbec.IMAG_RK3)
+            steps += chunk
+            d = c.diagnose()
+            out.append((steps, d["energy"], d["residual"]))
+            if time.time() - last > 20:
+                print(f"[warmstart] history {steps} steps: E {d['energy']:.10f} residual "
+                      f"{d['residual']:.3e}", flush=True)
+                last = time.time()
+        wall = time.time() - t0
+    return {"steps": steps, "wall_s": wall, "history": out}
+
+
+def first_reaching(hist, e_target, r_target=None):
+    """First recorded step count with E <= e_target (and residual <= r_target)."""
+    for st, e, r in hist:
+        if e <= e_target and (r_target is None or r <= r_target):
+            return st
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=2000)
@@ -166,6 +203,9 @@ def main():
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--test", type=float, nargs=5, default=[1.0, 0.8, 3000.0, 0.5, 0.0],
                     help="held-out parameters: w_perp w_z g Omega log_N")
+    ap.add_argument("--history", type=int, default=0,
+                    help="also run both starts this many steps without the stop rule and "
+                         "compare steps to equal energy along the way")
     ap.add_argument("--out", default="gpurun_out/warmstart")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
@@ -222,6 +262,20 @@ def main():
            "mlp_start_to_tol": r_ml_tol,
            "speedup_steps_to_tol": (r_tf["steps"] / max(1, r_ml_tol["steps"])
                                     if r_ml_tol and r_ml_tol["converged"] else None)}
+    if a.history:
+        h_tf = history(pr, a.history, start=tf_start(pr, 1.0, seed=0), tuning=tuning)
+        h_ml = history(pr, a.history, coarse=coarse, tuning=tuning)
+        e_best = min(e for _, e, _ in h_tf["history"] + h_ml["history"])
+        ladder = {}
+        for rel in (1e-2, 1e-3, 1e-4, 1e-5):
+            et = e_best * (1 + rel)
+            ladder[f"{rel:g}"] = {"E_target": et,
+                                  "tf_steps": first_reaching(h_tf["history"], et),
+                                  "mlp_steps": first_reaching(h_ml["history"], et)}
+            print(f"[warmstart] E <= E_min*(1+{rel:g}): TF {ladder[f'{rel:g}']['tf_steps']} "
+                  f"MLP {ladder[f'{rel:g}']['mlp_steps']} steps", flush=True)
+        out["history"] = {"E_min": e_best, "steps_to_energy": ladder,
+                          "tf": h_tf, "mlp": h_ml}
     tag = "_".join(f"{v:g}" for v in a.test)
     with open(os.path.join(a.out, f"warmstart_{tag}.json"), "w") as f:
         json.dump(out, f, indent=1)
