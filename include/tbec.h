@@ -3,7 +3,7 @@
  * The reference has no plugin interface: the hot path is the argument-less module
  * procedure get_rhs() (src/finite_diff.f90:8-136) operating on module globals
  * (src/cdata.f90:5-17) and the inline RK3 / imaginary-time array updates of
- * program run (src/run.f90:156-192), configured at compile time by cparam.local
+ * program run (src/run.f90:19-55), configured at compile time by cparam.local
  * (src/cparam.local:1-44).  libtbec replaces that path with a handle that owns the
  * device state; the host program (trapbec_amd/driver, Fortran via iso_c_binding, or
  * Python via ctypes) keeps the reference's driver role.
@@ -43,10 +43,10 @@ enum { TBEC_SUM_REFERENCE = 0, TBEC_SUM_COMPENSATED = 1, TBEC_SUM_CENTRED = 2 };
 
 /* Step modes. */
 enum {
-  TBEC_REAL_RK3 = 0,       /* src/run.f90:179-192 (dt complex, times (1 - i*gamma)) */
+  TBEC_REAL_RK3 = 0,       /* src/run.f90:42-55 (dt complex, times (1 - i*gamma)) */
   TBEC_IMAG_RK3 = 1,       /* RK3 with dt = (0,-dtau), FP64 renormalisation each step,
                               mu-shifted right-hand side (ground state is a fixed point) */
-  TBEC_IMAG_EULER_REF = 2  /* src/run.f90:157-159: psi += dt*0.5*rhs, no renormalisation */
+  TBEC_IMAG_EULER_REF = 2  /* src/run.f90:20-22: psi += dt*0.5*rhs, no renormalisation */
 };
 
 /* Run parameters: the cparam.local entries the stepper reads (src/cparam.local:1-44),
@@ -64,7 +64,7 @@ typedef struct {
   double omega_z;           /* frame term omega_z*i*(x D1y - y D1x) (src/finite_diff.f90:128) */
   double c0, gnl;
   double dt_re, dt_im;      /* complex dt (src/cparam.local:4) */
-  double gamma;             /* dissipation: real-time dt -> dt*(1 - i*gamma) (src/run.f90:179) */
+  double gamma;             /* dissipation: real-time dt -> dt*(1 - i*gamma) (src/run.f90:42) */
 } tbec_params;
 
 /* FP64 diagnostics of the current psi (dV = hx*hy*hz). */
@@ -102,7 +102,7 @@ int tbec_upload_psi_device(tbec_handle *h, const void *psi_dev);
 int tbec_download_psi_device(tbec_handle *h, void *psi_dev);
 
 /* Advance nsteps steps of the given mode (enqueued; returns before completion).
- * Real time advances *t_inout by Re(dt) per step (src/run.f90:192). */
+ * Real time advances *t_inout by Re(dt) per step (src/run.f90:55). */
 int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout);
 
 /* One get_rhs evaluation of the current psi (src/finite_diff.f90:8-136) into a host
@@ -180,7 +180,7 @@ int tbec_set_tuning(tbec_handle *h, int tile, int seg);
 enum { TBEC_BC_ZERO = 0, TBEC_BC_SOLID = 1, TBEC_BC_PERIODIC = 2 };
 int tbec_set_boundaries(tbec_handle *h, int boundaries, int periodicz);
 
-/* How the low-storage RK3 carries w between stages (src/run.f90:183-191).
+/* How the low-storage RK3 carries w between stages (src/run.f90:46-54).
  * stored = 1: the reference's scheme, w in its own array (10 HBM transfers of a complex
  * value per point and step).  stored = 0 (default): w is rebuilt pointwise from the two
  * psi buffers, w_{s-1} = (psi_s - psi_{s-1}) / B_{s-1} (8 transfers); the same RK3

@@ -73,7 +73,7 @@ def params(shape, L, harm_w, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3, gamma=0.0,
            dims=(1, 1), coords=(0, 0), two_dim=False, potential_off=None,
            boundaries="zero", periodicz=False):
     """Reference-unit parameters.  dt may be complex; the reference multiplies it by
-    (1 - i*gamma) before the real-time loop (src/run.f90:179)."""
+    (1 - i*gamma) before the real-time loop (src/run.f90:42)."""
     nx, ny, nz = shape
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     dtc = complex(dt) * complex(1.0, -gamma)

@@ -7,7 +7,7 @@
  * Part 1 (tbec_oracle_impl.h, instantiated for double and float) restates the
  * reference's stepper: get_rhs (src/finite_diff.f90:8-136), second_deriv and
  * cmplx_first_deriv (src/deriv.f90:12-25), the low-storage RK3 and imaginary Euler
- * updates (src/run.f90:157-192), coordinates (src/initial.f90:34-42) and Psi_info /
+ * updates (src/run.f90:20-55), coordinates (src/initial.f90:34-42) and Psi_info /
  * Psi_norm (src/diagnostic.f90:7-25).  Pinned against the reference build bit for
  * bit (FP64, -ffp-contract=off) and against closed forms in tests/test_oracle.py.
  *
@@ -109,7 +109,7 @@ void orc_energy_f64(const orc_params *p, double *psi_pad, double *rhs_scratch,
 }
 
 /* Renormalised imaginary-time RK3 step: the reference RK3 with dt = (0, -dtau)
- * (src/run.f90:183-192 with a complex dt, src/cparam.local:4), evaluated with the
+ * (src/run.f90:46-55 with a complex dt, src/cparam.local:4), evaluated with the
  * constant coefficient c0_eff = c0 + 2*mu_prev (the chemical potential of the previous
  * step, so the ground state is an exact fixed point of the step), then psi scaled so
  * that sum |psi|^2 dV = norm_target.  diag receives {N, E, mu, residual} of psi_n from

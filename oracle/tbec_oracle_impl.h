@@ -191,22 +191,22 @@ static void FN(psi_axpy)(const orc_params *p, T *psi_pad, const T *w_, int stage
       for (int k = 0; k < mz; ++k) {
         FN(cplx) *c = P + (long)(i + 3) * sx + (long)(j + 3) * sy + (k + 3);
         const FN(cplx) wv = W[((long)i * my + j) * mz + k];
-        if (stage == 0) {        /* Psi + (w/3.)  src/run.f90:185 */
+        if (stage == 0) {        /* Psi + (w/3.)  src/run.f90:48 */
           const FN(cplx) q = FN(cdiv_real)(wv, (T)3);
           c->re = c->re + q.re;
           c->im = c->im + q.im;
-        } else if (stage == 1) { /* Psi + w       src/run.f90:188 */
+        } else if (stage == 1) { /* Psi + w       src/run.f90:51 */
           c->re = c->re + wv.re;
           c->im = c->im + wv.im;
-        } else {                 /* Psi + 0.5*w   src/run.f90:191 */
+        } else {                 /* Psi + 0.5*w   src/run.f90:54 */
           c->re = c->re + (T)0.5 * wv.re;
           c->im = c->im + (T)0.5 * wv.im;
         }
       }
 }
 
-/* One low-storage RK3 step, src/run.f90:183-192.  dt is the complex step after the
- * dissipation factor dt*(1-gamma*i) (src/run.f90:179).  rhs is scratch.  Returns t+Re(dt). */
+/* One low-storage RK3 step, src/run.f90:46-55.  dt is the complex step after the
+ * dissipation factor dt*(1-gamma*i) (src/run.f90:42).  rhs is scratch.  Returns t+Re(dt). */
 double FN(orc_rk3_step)(const orc_params *p, T *psi_pad, T *w_, T *rhs_, double t) {
   const long n = (long)(p->nx / p->dims_x) * (p->ny / p->dims_y) * p->nz;
   const T dr = (T)p->dt_re, di = (T)p->dt_im;
@@ -236,7 +236,7 @@ double FN(orc_rk3_step)(const orc_params *p, T *psi_pad, T *w_, T *rhs_, double 
   return (double)((T)t + dr);
 }
 
-/* Reference imaginary-time step, src/run.f90:158-159: Psi = Psi + dt*0.5*rhs. */
+/* Reference imaginary-time step, src/run.f90:21-22: Psi = Psi + dt*0.5*rhs. */
 void FN(orc_euler_step)(const orc_params *p, T *psi_pad, T *rhs_, double t) {
   const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
   const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6);

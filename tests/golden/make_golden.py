@@ -62,7 +62,7 @@ def case_small(name, prec, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), steps=20,
 
 
 def case_euler(name="euler_f64_16", steps=20):
-    """Reference imaginary time (src/run.f90:156-170) from its own 'soliton' initial
+    """Reference imaginary time (src/run.f90:19-33) from its own 'soliton' initial
     state (setup_thomas_fermi, src/initial.f90:65-77,637-652): no restart file."""
     shape, box = (16, 16, 16), (16.0, 16.0, 16.0)
     harm = (4.0, 4.0, 4.0)   # TF_dense = 1 - pot stays positive over part of the box

@@ -28,7 +28,7 @@ def _stepper_from_fixture(f):
 
 @pytest.mark.parametrize("name", SMALL)
 def test_matches_reference_bitwise(name):
-    """RK3 (src/run.f90:179-192) over 20 steps equals the reference build bit for bit:
+    """RK3 (src/run.f90:42-55) over 20 steps equals the reference build bit for bit:
     FP64 and FP32, rotation on/off, dissipation, complex (imaginary) dt, an anisotropic
     grid, and a 4-rank run (decomposition invariance)."""
     f = np.load(os.path.join(GOLDEN, name + ".npz"))
@@ -50,7 +50,7 @@ def test_decomposition_invariance():
 
 
 def test_reference_euler_imaginary_time():
-    """Reference imaginary time (src/run.f90:156-170) from its own 'soliton' state
+    """Reference imaginary time (src/run.f90:19-33) from its own 'soliton' state
     (setup_thomas_fermi, src/initial.f90:637-652 + the sign flip at :65-77)."""
     f = np.load(os.path.join(GOLDEN, "euler_f64_16.npz"))
     box = tuple(float(b) for b in f["box"])
@@ -164,7 +164,7 @@ def test_potential_and_nonlinearity_single_point():
 
 def test_rk3_linear_stability_polynomial():
     """For a linear right-hand side (gnl = 0) one RK3 step is R(A) psi with
-    R(z) = 1 + z + z^2/2 + z^3/6 and A psi = (i dt/2) rhs(psi) (src/run.f90:183-191),
+    R(z) = 1 + z + z^2/2 + z^3/6 and A psi = (i dt/2) rhs(psi) (src/run.f90:46-54),
     so the A = (0,-2/3,-1), B = (1/3,1,1/2) coefficients are pinned."""
     n, L = 12, 6.0
     rng = np.random.default_rng(5)

@@ -5,8 +5,8 @@
 //         + (c0 - gnl*|psi|^2 - pot) psi,     pot = sum_a harm_w[a]*(x_a/L_a)^2
 // (src/finite_diff.f90:19-135, src/deriv.f90:12-25) and applies the stage update
 //   w <- A*w + c*rhs,  psi_out <- psi + B*w     (c = i*dt/2)
-// (src/run.f90:183-191), writing psi_out and w.  Variants: the reference imaginary
-// Euler update psi + (dt/2)*rhs (src/run.f90:159) and a rhs-only pass (test hook).
+// (src/run.f90:46-54), writing psi_out and w.  Variants: the reference imaginary
+// Euler update psi + (dt/2)*rhs (src/run.f90:22) and a rhs-only pass (test hook).
 //
 // Two ways to carry w between stages.  Stored (the reference's scheme): stages 1 and 2
 // write w, stages 2 and 3 read it: 10 HBM transfers of a complex value per point and

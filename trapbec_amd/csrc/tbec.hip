@@ -399,7 +399,7 @@ int tbec_reserve_partials(tbec_handle *h, int nb) {
 int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
   StageArgs a;
   const double dr = h->dt_re, di = h->dt_im;
-  // eye*0.5*dt (src/run.f90:184)
+  // eye*0.5*dt (src/run.f90:47)
   double cre = -0.5 * di, cim = 0.5 * dr;
   double A = 0.0, B = 1.0;
   if (kind == ST_RK1) { A = 0.0; B = 1.0 / 3.0; }
@@ -496,7 +496,7 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   h->hx = L.hx;
   h->hy = L.hy;
   h->hz = L.hz;
-  // dt*(1 - i*gamma) for real time (src/run.f90:179)
+  // dt*(1 - i*gamma) for real time (src/run.f90:42)
   h->dt_re = p->dt_re + p->gamma * p->dt_im;
   h->dt_im = p->dt_im - p->gamma * p->dt_re;
   h->PZ = L.PZ; h->PY = L.PY; h->PX = L.PX;

@@ -135,7 +135,7 @@ int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout) {
   rc = run_steps(h, nsteps, mode, nsteps >= kChunk ? kChunk : 0);
   if (rc) return rc;
   if (t_inout && mode == TBEC_REAL_RK3) {
-    // t = t + dt with the real part kept (src/run.f90:192), in the working precision
+    // t = t + dt with the real part kept (src/run.f90:55), in the working precision
     double t = *t_inout;
     for (int64_t i = 0; i < nsteps; ++i) {
       if (h->p.prec == TBEC_F32) t = (double)((float)t + (float)h->dt_re);

@@ -2,9 +2,9 @@
 !
 ! It keeps the reference's driver role (src/run.f90:1-81): run parameters (here a
 ! runtime namelist instead of the compile-time src/cparam.local), the initial state
-! (src/initial.f90), the imaginary-time stage (src/run.f90:156-172), the real-time loop
-! with its output cadence (src/run.f90:179-213), the per-shot diagnostics line
-! (src/run.f90:197-200), the var files and restart file (src/output.f90:71-108) and the
+! (src/initial.f90), the imaginary-time stage (src/run.f90:19-35), the real-time loop
+! with its output cadence (src/run.f90:42-76), the per-shot diagnostics line
+! (src/run.f90:60-63), the var files and restart file (src/output.f90:71-108) and the
 ! timing log (src/output.f90:111-118).  Every step runs on the GPU through libtbec.
 !
 !   tbec_run [namelist file]       (default: tbec.nml)

@@ -3,7 +3,7 @@
 This mirrors the reference's driver role (src/run.f90): parameters as in cparam.local
 (src/cparam.local:1-44), psi in/out as in reload_data / var_print
 (src/initial.f90:796-818, src/output.f90:71-108), the real-time RK3 loop
-(src/run.f90:179-192), the imaginary-time loop (src/run.f90:156-170) and the
+(src/run.f90:42-55), the imaginary-time loop (src/run.f90:19-33) and the
 diagnostics (src/diagnostic.f90:7-25).  All compute runs in libtbec's HIP kernels;
 there is no CPU fallback -- a missing library or GPU raises.
 """
