@@ -29,9 +29,15 @@ class OrcParams(C.Structure):
                 ("c0", C.c_double), ("gnl", C.c_double),
                 ("dt_re", C.c_double), ("dt_im", C.c_double),
                 ("potential_off", C.c_double), ("t_spindown", C.c_double),
-                ("boundaries", C.c_int), ("periodicz", C.c_int)]
+                ("boundaries", C.c_int), ("periodicz", C.c_int),
+                ("potential", C.c_int), ("lbeam", C.c_int), ("wbeam", C.c_double),
+                ("advective_vx", C.c_double), ("trap", C.c_void_p)]
 
 BOUNDARIES = {"zero": 0, "solid": 1, "periodic": 2}
+# the reference's `potential` names (src/finite_diff.f90:30-117) plus a user table
+POTENTIALS = {"zero": 0, "harmonic": 1, "harmonic_ring": 2, "harmonic_ring2": 3,
+              "barrier": 4, "box_trap": 5, "channel": 6, "quartic": 7, "sphere": 8,
+              "mobius": 9, "table": 10}
 
 
 def build(force: bool = False) -> str:
@@ -60,6 +66,10 @@ def lib():
             getattr(L, "orc_psi_info" + suf).argtypes = [P, vp, vp]
             getattr(L, "orc_velocity" + suf).argtypes = [P, vp, vp]
             getattr(L, "orc_qpressure" + suf).argtypes = [P, vp, vp]
+            getattr(L, "orc_potential" + suf).argtypes = [P, C.c_double, vp]
+            getattr(L, "orc_mobius_trap" + suf).argtypes = [P, vp]
+            getattr(L, "orc_imprint_vortex" + suf).argtypes = [P, vp, C.c_double,
+                                                                C.c_double, C.c_double]
         L.orc_sums_f64.argtypes = [P, vp, vp, vp]
         L.orc_energy_f64.argtypes = [P, vp, vp, C.c_double, vp]
         L.orc_imag_rk3_step_f64.argtypes = [P, vp, vp, vp, C.c_double, vp, vp]
@@ -71,9 +81,11 @@ def lib():
 
 def params(shape, L, harm_w, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3, gamma=0.0,
            dims=(1, 1), coords=(0, 0), two_dim=False, potential_off=None,
-           boundaries="zero", periodicz=False):
+           boundaries="zero", periodicz=False, potential="harmonic", lbeam=False,
+           wbeam=4.0, advective_vx=0.0, t_spindown=None):
     """Reference-unit parameters.  dt may be complex; the reference multiplies it by
-    (1 - i*gamma) before the real-time loop (src/run.f90:42)."""
+    (1 - i*gamma) before the real-time loop (src/run.f90:42).  For the "mobius" and
+    "table" potentials attach the Trap array with Stepper.set_trap."""
     nx, ny, nz = shape
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     dtc = complex(dt) * complex(1.0, -gamma)
@@ -81,7 +93,9 @@ def params(shape, L, harm_w, omega_z=0.0, c0=1.0, gnl=1.0, dt=1e-3, gamma=0.0,
                   Lx, Ly, Lz, (C.c_double * 3)(*harm_w), omega_z, c0, gnl,
                   dtc.real, dtc.imag,
                   np.finfo(np.float64).max if potential_off is None else potential_off,
-                  np.finfo(np.float64).max, BOUNDARIES[boundaries], int(periodicz))
+                  np.finfo(np.float64).max if t_spindown is None else t_spindown,
+                  BOUNDARIES[boundaries], int(periodicz), POTENTIALS[potential], int(lbeam),
+                  wbeam, advective_vx, None)
     return p
 
 
@@ -144,6 +158,30 @@ class Stepper:
         out = np.zeros((3,) + self.rhs.shape, dtype=self.psi.real.dtype)
         getattr(lib(), "orc_qpressure" + self.suf)(C.byref(self.p), _ptr(self.psi), _ptr(out))
         return out
+
+    def set_trap(self, trap: np.ndarray) -> None:
+        """Attach Trap(k,j,i) ([x, y, z], working precision) for the mobius/table kinds."""
+        self._trap = np.ascontiguousarray(trap, dtype=self.psi.real.dtype)
+        assert self._trap.shape == self.rhs.shape
+        self.p.trap = self._trap.ctypes.data
+
+    def mobius_trap(self) -> np.ndarray:
+        """The reference's mobius Trap table for this grid (src/initial.f90:160-196)."""
+        out = np.zeros(self.rhs.shape, dtype=self.psi.real.dtype)
+        getattr(lib(), "orc_mobius_trap" + self.suf)(C.byref(self.p), _ptr(out))
+        return out
+
+    def potential(self, t: float | None = None) -> np.ndarray:
+        """pot over the interior at time t (default: the stepper's t)."""
+        out = np.zeros(self.rhs.shape, dtype=self.psi.real.dtype)
+        getattr(lib(), "orc_potential" + self.suf)(C.byref(self.p),
+                                                   self.t if t is None else t, _ptr(out))
+        return out
+
+    def imprint_vortex(self, x0: float, y0: float, direction: float) -> None:
+        """force_random_points' imprint at a given centre and sense (src/force.f90:32-43)."""
+        getattr(lib(), "orc_imprint_vortex" + self.suf)(C.byref(self.p), _ptr(self.psi),
+                                                        x0, y0, direction)
 
     # FP64-only quantities with no reference counterpart
     def energy(self) -> dict:

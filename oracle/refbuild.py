@@ -42,6 +42,7 @@ DEFAULTS = dict(
     shots=10, nsteps=10, boundaries="zero", periodicz=False, init_cond="constant",
     potential="harmonic", harm_wx=256.0, harm_wy=256.0, harm_wz=256.0,
     potential_off=None, gamma=0.0, two_dim=False, omega_z=0.0, advective_vx=0.0,
+    lbeam=False, wbeam=4.0, t_spindown=None,
     imaginary_time=False, imag_nsteps=1, full_binary_output=True,
     imaginary_output=False, vel_print=False, qpress_print=False,
 )
@@ -66,6 +67,7 @@ def _lg(b: bool) -> str:
 
 def cparam_text(p: dict) -> str:
     off = "huge(1.)" if p["potential_off"] is None else _fl(p["potential_off"])
+    spin = "huge(1.)" if p["t_spindown"] is None else _fl(p["t_spindown"])
     return "\n".join([
         f"real  :: Lx={_fl(p['Lx'])}, Ly={_fl(p['Ly'])}, Lz={_fl(p['Lz'])}",
         f"integer, parameter :: Nx={p['Nx']}, Ny={p['Ny']}, Nz={p['Nz']}",
@@ -82,12 +84,12 @@ def cparam_text(p: dict) -> str:
         f"real, parameter :: gamma={_fl(p['gamma'])}",
         f"logical, parameter :: two_dim={_lg(p['two_dim'])}",
         "logical, parameter :: phase_imprint_on=.false.",
-        "logical, parameter :: lbeam=.false.",
+        f"logical, parameter :: lbeam={_lg(p['lbeam'])}",
         "logical, parameter :: add_pcurrent=.false.",
-        "real, parameter :: wbeam=4.",
+        f"real, parameter :: wbeam={_fl(p['wbeam'])}",
         f"real, parameter :: advective_vx={_fl(p['advective_vx'])}",
         f"real, parameter :: omega_z={_fl(p['omega_z'])}",
-        "real, parameter :: t_spindown=huge(1.)",
+        f"real, parameter :: t_spindown={spin}",
         "integer, parameter :: winding_number=1",
         "integer, parameter :: vor_ring_count=1",
         "real,parameter :: ring_rad=4.",

@@ -7,8 +7,10 @@
  * Part 1 (tbec_oracle_impl.h, instantiated for double and float) restates the
  * reference's stepper: get_rhs (src/finite_diff.f90:8-136), second_deriv and
  * cmplx_first_deriv (src/deriv.f90:12-25), the low-storage RK3 and imaginary Euler
- * updates (src/run.f90:20-55), coordinates (src/initial.f90:34-42) and Psi_info /
- * Psi_norm (src/diagnostic.f90:7-25).  Pinned against the reference build bit for
+ * updates (src/run.f90:20-55), coordinates (src/initial.f90:34-42), the potentials
+ * of get_rhs (src/finite_diff.f90:30-124) with the mobius Trap table
+ * (src/initial.f90:160-196), the vortex imprint of force_random_points
+ * (src/force.f90:32-53) and Psi_info / Psi_norm (src/diagnostic.f90:7-25).  Pinned against the reference build bit for
  * bit (FP64, -ffp-contract=off) and against closed forms in tests/test_oracle.py.
  *
  * Part 2 (below, FP64 only) restates quantities that have no reference code:
@@ -35,30 +37,68 @@ typedef struct {
   double potential_off, t_spindown;
   int boundaries;        /* 0 zero, 1 solid (mirror about the edge point), 2 periodic */
   int periodicz;         /* z periodic whatever the boundaries */
+  int potential;         /* ORC_POT_*: the `potential` select of src/finite_diff.f90:30-117 */
+  int lbeam;             /* laser beam term (src/finite_diff.f90:118-123) */
+  double wbeam;          /* its width */
+  double advective_vx;   /* 2 i vx D1x psi in place of the frame term (:126-127) */
+  const void *trap;      /* Trap(k,j,i) for ORC_POT_MOBIUS / ORC_POT_TABLE: interior
+                            layout, working precision */
 } orc_params;
+
+/* Potential kinds, in the order the C ABI numbers them. */
+enum {
+  ORC_POT_ZERO = 0, ORC_POT_HARMONIC = 1, ORC_POT_HARMONIC_RING = 2,
+  ORC_POT_HARMONIC_RING2 = 3, ORC_POT_BARRIER = 4, ORC_POT_BOX_TRAP = 5,
+  ORC_POT_CHANNEL = 6, ORC_POT_QUARTIC = 7, ORC_POT_SPHERE = 8, ORC_POT_MOBIUS = 9,
+  ORC_POT_TABLE = 10
+};
 
 #define T double
 #define SUF _f64
 #define HYPOT hypot
 #define ATAN2 atan2
+#define SQRT sqrt
+#define SIN sin
+#define COS cos
+#define ACOS acos
+#define EXP exp
+#define POW pow
 #define FLT_EPSILON_T DBL_EPSILON
 #include "tbec_oracle_impl.h"
 #undef T
 #undef SUF
 #undef HYPOT
 #undef ATAN2
+#undef SQRT
+#undef SIN
+#undef COS
+#undef ACOS
+#undef EXP
+#undef POW
 #undef FLT_EPSILON_T
 
 #define T float
 #define SUF _f32
 #define HYPOT hypotf
 #define ATAN2 atan2f
+#define SQRT sqrtf
+#define SIN sinf
+#define COS cosf
+#define ACOS acosf
+#define EXP expf
+#define POW powf
 #define FLT_EPSILON_T FLT_EPSILON
 #include "tbec_oracle_impl.h"
 #undef T
 #undef SUF
 #undef HYPOT
 #undef ATAN2
+#undef SQRT
+#undef SIN
+#undef COS
+#undef ACOS
+#undef EXP
+#undef POW
 #undef FLT_EPSILON_T
 
 /* ---------------- Part 2: FP64 quantities with no reference counterpart --------- */

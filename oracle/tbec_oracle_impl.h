@@ -116,12 +116,234 @@ void FN(orc_fill_ghosts)(const orc_params *p, T *psi_pad) {
   }
 }
 
-/* get_rhs, src/finite_diff.f90:8-136, for the harmonic potential (:33-34), the
- * rotating frame (:128-130) and the nonlinearity (:132), with zero ghosts
- * (src/ghost.f90:55,88,121,154,179-181: the padded halo is never written).
+/* Time-dependent scalars of the potential select (src/finite_diff.f90:30-124), in the
+ * working precision: s = max(0, 5 - 0.1 t) (ring, ring2, barrier openings), b =
+ * 5 sin(0.1 t) (box_trap tilt) and the laser centre (las_freq = 0). */
+typedef struct { T s, b, las_x, las_y, t; } FN(pot_time);
+
+static FN(pot_time) FN(pot_time_of)(T t) {
+  FN(pot_time) q;
+  const T m = (T)5 - (T)0.1 * t;
+  q.s = (T)0 > m ? (T)0 : m;                              /* max(0., 5.-0.1*t) */
+  q.b = (T)5 * SIN((T)0.1 * t);                           /* 5*sin(0.1*t) */
+  const T lf = (T)0;                                      /* las_freq=0.0 */
+  q.las_x = ((T)70 * COS(lf * t)) * ((T)1 - SIN(lf * t)); /* :120 */
+  q.las_y = ((T)30 * SIN(lf * t)) * COS(lf * t);          /* :121 */
+  q.t = t;
+  return q;
+}
+
+/* pot at interior point (i, j, k) (1-based, this block), src/finite_diff.f90:30-124,
+ * operand for operand.  Integer powers x**n are n-1 multiplications left to right
+ * (x*x, r*r*r*r*r*r*r*r), as amdflang emits them; real powers (harmonic_ring2's
+ * pow = 6.) call pow. */
+static T FN(pot_at)(const orc_params *p, int i, int j, int k, const T *xx, const T *yy,
+                    const T *zz, const FN(pot_time) *q) {
+  const T x = xx[i - 1], y = yy[j - 1], z = zz[k - 1];
+  const T Lx = (T)p->Lx, Ly = (T)p->Ly, Lz = (T)p->Lz;
+  const T wx = (T)p->harm_w[0], wy = (T)p->harm_w[1], wz = (T)p->harm_w[2];
+  T pot = (T)0;
+  switch (p->potential) {
+    case ORC_POT_ZERO:
+      break;
+    case ORC_POT_HARMONIC: {                                  /* :33-34 */
+      const T ax = x / Lx, ay = y / Ly, az = z / Lz;
+      pot = wx * (ax * ax) + wy * (ay * ay) + wz * (az * az);
+      break;
+    }
+    case ORC_POT_HARMONIC_RING: {                             /* :35-40 */
+      const T ax = x / Lx, ay = y / Lx;
+      const T r = SQRT(ax * ax + ay * ay);
+      const T d = r - (T)0.3;
+      pot = wx * (d * d);
+      if (fabs((double)(r - (T)0.3)) < (double)(T)0.01) pot = q->s;
+      break;
+    }
+    case ORC_POT_HARMONIC_RING2: {                            /* :41-65 */
+      const T pw = (T)6;
+      const T ax = x / Lx;
+      const T a1 = POW((T)0.25, pw);
+      const T u1 = POW(ax, pw), v1 = POW(y / Ly - (T)0.2, pw), v2 = POW(y / Ly + (T)0.2, pw);
+      T r1 = POW((T)fabs((double)((a1 - (T)0.15 * u1) - (T)4 * v1)), (T)1 / pw);
+      T r2 = POW((T)fabs((double)((a1 - (T)0.15 * u1) - (T)4 * v2)), (T)1 / pw);
+      r1 = r1 < (T)0.2495 ? (T)0 : (T)5;
+      r2 = r2 < (T)0.2495 ? (T)0 : (T)5;
+      pot = r1 * r2;
+      if (pot < (T)0) pot = (T)0;
+      if (fabs((double)(y / Ly)) < (double)(T)0.01 && pot < (T)0.01) pot = q->s;
+      break;
+    }
+    case ORC_POT_BARRIER:                                     /* :66-71 */
+      pot = (T)0;
+      if (fabs((double)(y / Ly)) > (double)(T)0.3) pot = (T)5;
+      if (fabs((double)(y / Ly)) < (double)(T)0.01) pot = q->s;
+      break;
+    case ORC_POT_BOX_TRAP: {                                  /* :72-78 */
+      pot = (T)10;
+      const T r = SQRT(x * x + y * y);
+      if (r < Lx / (T)3 && fabs((double)z) < (double)(Lz / (T)2.5)) pot = (T)0;
+      pot = pot + (q->b * z) / Lz;
+      break;
+    }
+    case ORC_POT_CHANNEL: {                                   /* :79-96 */
+      pot = (T)0;
+      if (fabs((double)y) > (double)(((T)0.75 * Ly) / (T)2) ||
+          fabs((double)z) > (double)(((T)0.75 * Lz) / (T)2))
+        pot = (T)5;
+      const T hxg = Lx / (T)p->nx, hyg = Ly / (T)p->ny;
+      for (int l1 = -1; l1 <= 1; ++l1)
+        if (fabs((double)x) < (double)hxg &&
+            fabs((double)(y - ((T)l1 * Ly) / (T)5)) < (double)hyg)
+          pot = (T)7.5;
+      for (int l1 = -1; l1 <= 1; ++l1)
+        if (fabs((double)x) < (double)hxg &&
+            fabs((double)(z - ((T)l1 * Lz) / (T)5)) < (double)hyg)
+          pot = (T)7.5;
+      break;
+    }
+    case ORC_POT_QUARTIC: {                                   /* :97-99 */
+      const T ax = x / Lx, ay = y / Ly;
+      const T r = SQRT(ax * ax + ay * ay);
+      /* r**8: amdflang multiplies out left to right (pinned by the reference build) */
+      const T r8 = r * r * r * r * r * r * r * r;
+      pot = wx * r8;
+      break;
+    }
+    case ORC_POT_SPHERE: {                                    /* :100-111 */
+      const T theta = ATAN2(y, x);
+      const T ax = x / Lx, ay = y / Ly, az = z / Lz;
+      const T r1 = SQRT((ax * ax + ay * ay) + az * az);
+      const T phi = ACOS(az / r1);
+      const T lim = ((T)0.2 + (T)0.01 * SIN((T)20 * phi)) +
+                    (T)0.01 * COS((T)30 * theta + (T)0.03);
+      pot = r1 < lim ? (T)10 : (T)0;
+      break;
+    }
+    case ORC_POT_MOBIUS:                                      /* :112-116 */
+    case ORC_POT_TABLE: {
+      const int my = p->ny / p->dims_y, mz = p->nz;
+      pot = ((const T *)p->trap)[((long)(i - 1) * my + (j - 1)) * mz + (k - 1)];
+      if (p->potential == ORC_POT_MOBIUS && fabs((double)y) < 1.0 &&
+          fabs((double)(x - (T)25)) < 1.0)
+        pot = pot + (T)10;
+      break;
+    }
+    default:
+      break;
+  }
+  (void)wy;
+  (void)wz;
+  if (p->lbeam) {                                             /* :118-123 */
+    const T wb = (T)p->wbeam;
+    const T u = (x - q->las_x) / wb, v = (y - q->las_y) / wb;
+    pot = pot + (T)20 * EXP(-(u * u) - v * v);
+  }
+  if (q->t > (T)p->potential_off) pot = (T)0;                 /* :124 */
+  return pot;
+}
+
+/* The potential over this block's interior at time t (interior layout, z fastest). */
+void FN(orc_potential)(const orc_params *p, double t, T *out) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  T *xx = malloc(sizeof(T) * mx), *yy = malloc(sizeof(T) * my), *zz = malloc(sizeof(T) * mz);
+  FN(coords)(p, xx, yy, zz);
+  const FN(pot_time) q = FN(pot_time_of)((T)t);
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= mx; ++i)
+    for (int j = 1; j <= my; ++j)
+      for (int k = 1; k <= mz; ++k)
+        out[((long)(i - 1) * my + (j - 1)) * mz + (k - 1)] =
+            FN(pot_at)(p, i, j, k, xx, yy, zz, &q);
+  free(xx);
+  free(yy);
+  free(zz);
+}
+
+/* The mobius Trap table, src/initial.f90:160-196: 200 x 200 points of the strip
+ * ((25 + 5 r cos(th/2)) cos th, (25 + 5 r cos(th/2)) sin th, 5 r sin(th/2)),
+ * Trap = sum of exp(-1.5 |x - m|^2) over the strip in (ii1, ii2) order, capped at 3
+ * after every term, then Trap = max(Trap) - Trap.  This block's max only (one block). */
+void FN(orc_mobius_trap)(const orc_params *p, T *trap) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const T pi = (T)3.14159265359;   /* src/cdata.f90:13 */
+  T *m = malloc(sizeof(T) * 200 * 200 * 3);
+  for (int i = 1; i <= 200; ++i)
+    for (int j = 1; j <= 200; ++j) {
+      const T r = ((T)(i - 1) * (T)2) / (T)200 - (T)1;
+      const T th = (((T)(j - 1) * (T)2) * pi) / (T)200;
+      T *e = m + ((long)(i - 1) * 200 + (j - 1)) * 3;
+      e[0] = ((T)25 + ((T)5 * r) * COS(th / (T)2)) * COS(th);
+      e[1] = ((T)25 + ((T)5 * r) * COS(th / (T)2)) * SIN(th);
+      e[2] = ((T)5 * r) * SIN(th / (T)2);
+    }
+  T *xx = malloc(sizeof(T) * mx), *yy = malloc(sizeof(T) * my), *zz = malloc(sizeof(T) * mz);
+  FN(coords)(p, xx, yy, zz);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 1; i <= mx; ++i)
+    for (int j = 1; j <= my; ++j)
+      for (int k = 1; k <= mz; ++k) {
+        T acc = (T)0;
+        for (int a = 0; a < 200 * 200; ++a) {
+          const T *e = m + (long)a * 3;
+          const T dx = xx[i - 1] - e[0], dy = yy[j - 1] - e[1], dz = zz[k - 1] - e[2];
+          acc = acc + EXP(-((T)1.5 * (dx * dx)) - (T)1.5 * (dy * dy) - (T)1.5 * (dz * dz));
+          if (acc > (T)3) acc = (T)3;
+        }
+        trap[((long)(i - 1) * my + (j - 1)) * mz + (k - 1)] = acc;
+      }
+  const long n = (long)mx * my * mz;
+  T mxv = trap[0];
+  for (long q = 1; q < n; ++q) mxv = trap[q] > mxv ? trap[q] : mxv;
+  for (long q = 0; q < n; ++q) trap[q] = mxv - trap[q];
+  free(m);
+  free(xx);
+  free(yy);
+  free(zz);
+}
+
+/* The vortex imprint of force_random_points at a given centre and sense,
+ * src/force.f90:32-53: psi *= pade(r) exp(i dir theta) about (x0, y0), with
+ * pade(r) = sqrt(r^2 (c1 + c2 r^2) / (1 + c3 r^2 + c2 r^4)), c1 = 11/32. */
+void FN(orc_imprint_vortex)(const orc_params *p, T *psi_pad, double x0, double y0,
+                            double dir) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6);
+  T *xx = malloc(sizeof(T) * mx), *yy = malloc(sizeof(T) * my), *zz = malloc(sizeof(T) * mz);
+  FN(coords)(p, xx, yy, zz);
+  const T vx = (T)x0, vy = (T)y0, d = (T)dir;
+  const T c1 = (T)11 / (T)32;
+  const T c3 = ((T)5 - (T)32 * c1) / ((T)48 - (T)192 * c1);
+  const T c2 = c1 * (c3 - (T)0.25);
+  FN(cplx) *P = (FN(cplx) *)psi_pad;
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= mx; ++i)
+    for (int j = 1; j <= my; ++j) {
+      const T ddx = xx[i - 1] - vx, ddy = yy[j - 1] - vy;
+      const T r = SQRT(ddx * ddx + ddy * ddy);
+      const T theta = ATAN2(ddy, ddx);
+      const T r2 = r * r, r4 = r * r * r * r;
+      const T func = SQRT((r2 * (c1 + c2 * r2)) / (((T)1 + c3 * r2) + c2 * r4));
+      const T ang = d * theta;
+      const T fr = func * COS(ang), fi = func * SIN(ang);
+      for (int k = 1; k <= mz; ++k) {
+        FN(cplx) *c = P + (long)(i + 2) * sx + (long)(j + 2) * sy + (k + 2);
+        const T a = c->re, b = c->im;
+        c->re = a * fr - b * fi;
+        c->im = a * fi + b * fr;
+      }
+    }
+  free(xx);
+  free(yy);
+  free(zz);
+}
+
+/* get_rhs, src/finite_diff.f90:8-136: the Laplacian, the potential select (:30-124,
+ * pot_at), the advective (:126-127) or rotating-frame (:128-130) term and the
+ * nonlinearity (:132), after ghost_comm (src/ghost.f90:27-192, orc_fill_ghosts).
  * Generalised by two coefficients that equal 1 in the reference:
  *   rhs += (c0 - gnl*abs(psi**2) - pot)*psi
- * (gnl*x == x exactly when gnl == 1, so the reference arithmetic is unchanged). */
+ * (gnl*x == x exactly when gnl == 1, so the reference arithmetic is unchanged).  t, the
+ * switch-off times and epsilon(0.) are compared in the working precision. */
 void FN(orc_get_rhs)(const orc_params *p, T *psi_pad, T *rhs_out, double t) {
   FN(orc_fill_ghosts)(p, psi_pad);
   const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
@@ -131,11 +353,13 @@ void FN(orc_get_rhs)(const orc_params *p, T *psi_pad, T *rhs_out, double t) {
   FN(coords)(p, xx, yy, zz);
   const FN(cplx) *P = (const FN(cplx) *)psi_pad;
   FN(cplx) *R = (FN(cplx) *)rhs_out;
-  const T wx = (T)p->harm_w[0], wy = (T)p->harm_w[1], wz = (T)p->harm_w[2];
-  const T Lx = (T)p->Lx, Ly = (T)p->Ly, Lz = (T)p->Lz;
   const T om = (T)p->omega_z, c0 = (T)p->c0, gnl = (T)p->gnl;
-  const int rot = (fabs((double)om) > (double)FLT_EPSILON_T) && (t < p->t_spindown);
-  const int pot_on = !(t > p->potential_off);
+  const T tt = (T)t;
+  const T vx = (T)p->advective_vx;
+  const int adv = fabs((double)vx) > (double)FLT_EPSILON_T;
+  const int rot = !adv && (fabs((double)om) > (double)FLT_EPSILON_T) &&
+                  (tt < (T)p->t_spindown);
+  const FN(pot_time) ptime = FN(pot_time_of)(tt);
 #pragma omp parallel for schedule(static)
   for (int i = 1; i <= mx; ++i) {
     for (int j = 1; j <= my; ++j) {
@@ -151,12 +375,14 @@ void FN(orc_get_rhs)(const orc_params *p, T *psi_pad, T *rhs_out, double t) {
           r.re = r.re + q.re;
           r.im = r.im + q.im;
         }
-        T pot = (T)0;
-        if (pot_on) {
-          const T ax = xx[i - 1] / Lx, ay = yy[j - 1] / Ly, az = zz[k - 1] / Lz;
-          pot = wx * (ax * ax) + wy * (ay * ay) + wz * (az * az);
-        }
-        if (rot) {
+        const T pot = FN(pot_at)(p, i, j, k, xx, yy, zz, &ptime);
+        if (adv) {
+          /* 2*eye*advective_vx*D1x: (0, 2 vx) * (d.re, d.im) */
+          const FN(cplx) dx1 = FN(d1)(c, sx, hx);
+          const T v2 = (T)2 * vx;
+          r.re = r.re + ((T)0 * dx1.re - v2 * dx1.im);
+          r.im = r.im + ((T)0 * dx1.im + v2 * dx1.re);
+        } else if (rot) {
           const FN(cplx) dy1 = FN(d1)(c, sy, hy), dx1 = FN(d1)(c, sx, hx);
           const T pr = xx[i - 1] * dy1.re - yy[j - 1] * dx1.re;
           const T pi_ = xx[i - 1] * dy1.im - yy[j - 1] * dx1.im;

@@ -117,6 +117,41 @@ def case_diag(name, prec, nprocs=1, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), 
                         prec=prec, box=np.array(box), nprocs=nprocs)
 
 
+def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), steps=20,
+             t0=0.0, omega_z=0.0, potential_off=None, t_spindown=None, advective_vx=0.0,
+             lbeam=False, wbeam=4.0, harm=None, seed=17, constant_ic=False):
+    """The potential select of get_rhs (src/finite_diff.f90:30-124) with the laser term,
+    the switch-off times and the advective term: real-time RK3 from a seeded state
+    restarted at t0 (so the time-dependent openings and tilts are exercised), or, for
+    the mobius trap (whose Trap table the reference builds only on a fresh start,
+    src/initial.f90:54-57,159-196), from the reference's own 'constant' state."""
+    harm = tuple(L * L for L in box) if harm is None else harm
+    d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                       Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=steps,
+                       omega_z=omega_z, potential=potential, potential_off=potential_off,
+                       t_spindown=t_spindown, advective_vx=advective_vx, lbeam=lbeam,
+                       wbeam=wbeam, harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
+    cdt = np.complex128 if prec == 64 else np.complex64
+    rdt = np.float64 if prec == 64 else np.float32
+    if constant_ic:
+        c = rdt(1) / np.sqrt(rdt(2))       # cmplx(1./sqrt(2.),1./sqrt(2.)), initial.f90:63
+        psi0 = np.full(shape, c + 1j * c, dtype=cdt)
+        r = refbuild.run(d, None, prec=prec, nprocs=1, shape=shape)
+    else:
+        psi0 = small_ic(shape, box, seed).astype(cdt)
+        r = refbuild.run(d, psi0, prec=prec, nprocs=1, t0=t0)
+    inf = np.inf
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi0=psi0, psi=r["psi"],
+                        t=r["t"], t0=t0, steps=steps, prec=prec, box=np.array(box),
+                        harm_w=np.array(harm), omega_z=omega_z, gamma=0.0,
+                        dt=np.complex128(1e-3), nprocs=1, boundaries="zero",
+                        periodicz=False, potential=potential,
+                        potential_off=inf if potential_off is None else potential_off,
+                        t_spindown=inf if t_spindown is None else t_spindown,
+                        advective_vx=advective_vx, lbeam=lbeam, wbeam=wbeam)
+    return r
+
+
 CASES = {
     "rk3_f64_16": lambda: case_small("rk3_f64_16", 64),
     "rk3_f64_16_rot": lambda: case_small("rk3_f64_16_rot", 64, omega_z=-1.4),
@@ -145,6 +180,28 @@ CASES = {
     "diag_f32_16": lambda: case_diag("diag_f32_16", 32),
     "diag_f64_16_np4": lambda: case_diag("diag_f64_16_np4", 64, nprocs=4),
     "c1_f64_64": case_c1,
+    "pot_ring_f64": lambda: case_pot("pot_ring_f64", 64, "harmonic_ring", t0=49.99),
+    "pot_ring2_f32": lambda: case_pot("pot_ring2_f32", 32, "harmonic_ring2", t0=49.995),
+    "pot_barrier_f64": lambda: case_pot("pot_barrier_f64", 64, "barrier",
+                                        potential_off=0.0105),
+    "pot_laser_f64": lambda: case_pot("pot_laser_f64", 64, "zero", shape=(48, 16, 16),
+                                      box=(160.0, 32.0, 16.0), lbeam=True, wbeam=8.0),
+    "pot_box_f64": lambda: case_pot("pot_box_f64", 64, "box_trap", t0=15.0, omega_z=-1.4,
+                                    t_spindown=15.0105),
+    "pot_channel_f32": lambda: case_pot("pot_channel_f32", 32, "channel", shape=(20, 20, 20),
+                                        advective_vx=0.3),
+    "pot_quartic_f32": lambda: case_pot("pot_quartic_f32", 32, "quartic", shape=(16, 16, 16)),
+    "pot_quartic_f64": lambda: case_pot("pot_quartic_f64", 64, "quartic", shape=(16, 16, 16)),
+    "pot_sphere_f64": lambda: case_pot("pot_sphere_f64", 64, "sphere"),
+    "pot_sphere_f32": lambda: case_pot("pot_sphere_f32", 32, "sphere"),
+    "pot_mobius_f64": lambda: case_pot("pot_mobius_f64", 64, "mobius", shape=(32, 32, 16),
+                                       box=(64.0, 64.0, 16.0), steps=10, constant_ic=True),
+    "pot_harm_switch_f64": lambda: case_pot("pot_harm_switch_f64", 64, "harmonic",
+                                            shape=(16, 16, 16), omega_z=-1.4,
+                                            potential_off=0.0105, t_spindown=0.0155),
+    "pot_advective_f64": lambda: case_pot("pot_advective_f64", 64, "harmonic",
+                                          shape=(16, 16, 16), omega_z=-1.4,
+                                          advective_vx=0.5),
 }
 
 if __name__ == "__main__":

@@ -20,10 +20,23 @@ def _stepper_from_fixture(f):
     box = tuple(float(b) for b in f["box"])
     bc = str(f["boundaries"]) if "boundaries" in f else "zero"
     pz = bool(f["periodicz"]) if "periodicz" in f else False
+    def opt(key, default, conv=float):
+        if key not in f:
+            return default
+        v = conv(f[key])
+        return None if (conv is float and np.isinf(v)) else v
     p = oracle.params(f["psi0"].shape, box, tuple(float(h) for h in f["harm_w"]),
                       omega_z=float(f["omega_z"]), dt=complex(f["dt"]),
-                      gamma=float(f["gamma"]), boundaries=bc, periodicz=pz)
-    return oracle.Stepper(p, f["psi0"], prec=prec), prec
+                      gamma=float(f["gamma"]), boundaries=bc, periodicz=pz,
+                      potential=opt("potential", "harmonic", str),
+                      potential_off=opt("potential_off", None),
+                      t_spindown=opt("t_spindown", None),
+                      advective_vx=opt("advective_vx", 0.0),
+                      lbeam=opt("lbeam", False, bool), wbeam=opt("wbeam", 4.0))
+    s = oracle.Stepper(p, f["psi0"], prec=prec, t=opt("t0", 0.0))
+    if "potential" in f and str(f["potential"]) == "mobius":
+        s.set_trap(s.mobius_trap())
+    return s, prec
 
 
 @pytest.mark.parametrize("name", SMALL)
@@ -31,6 +44,29 @@ def test_matches_reference_bitwise(name):
     """RK3 (src/run.f90:42-55) over 20 steps equals the reference build bit for bit:
     FP64 and FP32, rotation on/off, dissipation, complex (imaginary) dt, an anisotropic
     grid, and a 4-rank run (decomposition invariance)."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    s, prec = _stepper_from_fixture(f)
+    s.rk3(int(f["steps"]))
+    got, ref = s.interior, f["psi"]
+    assert got.dtype == ref.dtype
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), \
+        f"{np.sum(got != ref)} points differ, max {np.abs(got - ref).max()}"
+    assert s.t == pytest.approx(float(f["t"]), rel=1e-6 if prec == 32 else 1e-15)
+
+
+POT = ["pot_ring_f64", "pot_ring2_f32", "pot_barrier_f64", "pot_laser_f64", "pot_box_f64",
+       "pot_channel_f32", "pot_quartic_f32", "pot_quartic_f64", "pot_sphere_f64",
+       "pot_sphere_f32", "pot_mobius_f64", "pot_harm_switch_f64", "pot_advective_f64"]
+
+
+@pytest.mark.parametrize("name", POT)
+def test_potentials_match_reference_bitwise(name):
+    """The potential select (src/finite_diff.f90:30-124): ring openings closing at
+    t = 50, the barrier switched off mid-run, the laser beam, the box trap's sin(0.1 t)
+    tilt with the frame term spun down mid-run, the channel with the advective term,
+    quartic (r**8), sphere, the mobius Trap table (src/initial.f90:160-196) and the
+    harmonic trap with both switch-off times: 20 (mobius 10) RK3 steps equal the
+    reference build bit for bit."""
     f = np.load(os.path.join(GOLDEN, name + ".npz"))
     s, prec = _stepper_from_fixture(f)
     s.rk3(int(f["steps"]))
