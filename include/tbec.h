@@ -102,7 +102,10 @@ int tbec_upload_psi_device(tbec_handle *h, const void *psi_dev);
 int tbec_download_psi_device(tbec_handle *h, void *psi_dev);
 
 /* Advance nsteps steps of the given mode (enqueued; returns before completion).
- * Real time advances *t_inout by Re(dt) per step (src/run.f90:55). */
+ * Real time advances the handle's time by Re(dt) per step in the working precision
+ * (src/run.f90:55); t_inout, when not null, sets that time first and receives it after.
+ * Each step evaluates the potential, the frame-term switch and the switch-off at the
+ * time it starts (get_rhs reads t, src/finite_diff.f90:124,128). */
 int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout);
 
 /* One get_rhs evaluation of the current psi (src/finite_diff.f90:8-136) into a host
@@ -179,6 +182,56 @@ int tbec_set_tuning(tbec_handle *h, int tile, int seg);
  * before every stage (a pass over the 6 faces).  Drops captured graphs. */
 enum { TBEC_BC_ZERO = 0, TBEC_BC_SOLID = 1, TBEC_BC_PERIODIC = 2 };
 int tbec_set_boundaries(tbec_handle *h, int boundaries, int periodicz);
+
+/* ---- physics beyond the harmonic trap -------------------------------------------
+ * The reference selects these in cparam.local and evaluates them inside get_rhs
+ * (src/finite_diff.f90:30-130).  The potential kinds are the reference's `potential`
+ * names in order, plus TBEC_POT_TABLE (a caller-supplied Trap table, no reference
+ * counterpart; the reference's own Trap is the mobius one). */
+enum {
+  TBEC_POT_ZERO = 0,            /* 'zero'           :31-32 */
+  TBEC_POT_HARMONIC = 1,        /* 'harmonic'       :33-34 (default; harm_w of the params) */
+  TBEC_POT_HARMONIC_RING = 2,   /* 'harmonic_ring'  :35-40, opening max(0, 5 - 0.1 t) */
+  TBEC_POT_HARMONIC_RING2 = 3,  /* 'harmonic_ring2' :41-65 */
+  TBEC_POT_BARRIER = 4,         /* 'barrier'        :66-71 */
+  TBEC_POT_BOX_TRAP = 5,        /* 'box_trap'       :72-78, tilt 5 sin(0.1 t) z/Lz */
+  TBEC_POT_CHANNEL = 6,         /* 'channel'        :79-96 */
+  TBEC_POT_QUARTIC = 7,         /* 'quartic'        :97-99 */
+  TBEC_POT_SPHERE = 8,          /* 'sphere'         :100-111 */
+  TBEC_POT_MOBIUS = 9,          /* 'mobius'         :112-116, Trap of src/initial.f90:160-196 */
+  TBEC_POT_TABLE = 10           /* pot = the table given to tbec_set_potential_table */
+};
+typedef struct {
+  int32_t potential;        /* TBEC_POT_* */
+  int32_t lbeam;            /* add the laser beam 20 exp(-|(x,y) - (70,0)|^2 / wbeam^2)
+                               (src/finite_diff.f90:118-123) */
+  double wbeam;             /* beam width */
+  double potential_off;     /* pot = 0 while t > potential_off (:124); HUGE_VAL: never */
+  double t_spindown;        /* frame term only while t < t_spindown (:128); HUGE_VAL: never */
+  double advective_vx;      /* |vx| > epsilon: 2 i vx D1x psi replaces the frame term
+                               (:126-127) */
+} tbec_physics;
+/* Select the physics.  Times are compared in the working precision, as the reference
+ * compares its `real` t.  TBEC_POT_MOBIUS builds the Trap table on the device here.
+ * Harmonic or zero potentials without laser or advective term keep the analytic
+ * stage kernel; every other choice reads a tabulated potential, refreshed when the
+ * potential changes with t (one extra pass per step for box_trap, none while the
+ * table is constant).  Drops captured graphs. */
+int tbec_set_physics(tbec_handle *h, const tbec_physics *ph);
+/* The Trap table for TBEC_POT_TABLE (nx*ny*nz reals of the working precision, z
+ * fastest, host buffer). */
+int tbec_set_potential_table(tbec_handle *h, const void *trap_host);
+/* pot over the interior at time t (nx*ny*nz reals, z fastest, host buffer); synchronises. */
+int tbec_potential(tbec_handle *h, double t, void *pot_host);
+/* The handle's time t: real-time steps start from it and advance it; imaginary-time
+ * steps and the diagnostics evaluate the potential at it (0 after creation, as the
+ * reference's t, src/cdata.f90:17).  tbec_step's t_inout, when given, sets it first. */
+int tbec_set_time(tbec_handle *h, double t);
+/* Multiply psi by a straight vortex line along z at (x0, y0) with sense dir (+1 or -1):
+ * psi *= pade(r) exp(i dir theta), the imprint of force_random_points
+ * (src/force.f90:32-53; the reference draws x0, y0 uniform in [-L/4, L/4] and dir
+ * every force_freq steps, src/run.f90:70-74). */
+int tbec_imprint_vortex(tbec_handle *h, double x0, double y0, double dir);
 
 /* How the low-storage RK3 carries w between stages (src/run.f90:46-54).
  * stored = 1: the reference's scheme, w in its own array (10 HBM transfers of a complex

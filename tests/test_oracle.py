@@ -432,3 +432,37 @@ def test_diagnostics_closed_forms():
     assert errs[1] < 1e-2
     assert 3.5 < errs[0] / errs[1] < 4.5          # second order (low_first_deriv)
     assert h < 0.3
+
+
+def test_imprint_vortex_closed_forms():
+    """force_random_points' imprint (src/force.f90:32-53) on a uniform state: the
+    magnitude is Berloff's Pade approximant of the GP vortex core, f^2 = r^2 (a1 + a2 r^2)
+    / (1 + b1 r^2 + a2 r^4) with the published a1 = 0.34375, b1 = 1/3, a2 = 11/384
+    (N. G. Berloff, J. Phys. A 37, 1617 (2004)); the phase winds once (sense dir) around
+    exactly one plaquette per z-plane at the centre; +1 then -1 at the same centre leaves
+    the real factor f^2 and no phase."""
+    shape, box = (24, 20, 8), (16.0, 14.0, 4.0)
+    x0, y0 = 1.3, -2.1
+    psi0 = np.ones(shape, dtype=np.complex128)
+    p = oracle.params(shape, box, (1.0, 1.0, 1.0))
+    s = oracle.Stepper(p, psi0)
+    s.imprint_vortex(x0, y0, 1.0)
+    X, Y, _ = initial.mesh(shape, box)
+    r2 = (X - x0) ** 2 + (Y - y0) ** 2
+    a1, b1, a2 = 0.34375, 1.0 / 3.0, 11.0 / 384.0
+    f = np.sqrt(r2 * (a1 + a2 * r2) / (1 + b1 * r2 + a2 * r2 * r2))
+    got = s.interior
+    assert np.abs(np.abs(got) - f).max() < 1e-13
+    v = s.vortices(1e-6, 0)
+    assert len(v) == shape[2] and set(v[:, 3]) == {1}
+    h = (box[0] / shape[0], box[1] / shape[1])
+    cx = (v[:, 0] + 1) * h[0] - box[0] / 2          # plaquette centre (corners at cells)
+    cy = (v[:, 1] + 1) * h[1] - box[1] / 2
+    assert np.all(np.abs(cx - x0) < h[0]) and np.all(np.abs(cy - y0) < h[1])
+    s.imprint_vortex(x0, y0, -1.0)
+    back = s.interior
+    assert np.abs(back.imag).max() < 1e-14
+    assert np.abs(back.real - f * f).max() < 1e-13
+    s2 = oracle.Stepper(p, psi0)
+    s2.imprint_vortex(x0, y0, -1.0)
+    assert set(s2.vortices(1e-6, 0)[:, 3]) == {-1}

@@ -52,6 +52,14 @@
 namespace tbec {
 
 enum SumMode { SUM_REF = 0, SUM_COMP = 1, SUM_CENTRED = 2 };
+// Physics variants of the stage kernel (a template parameter):
+//   PH_HARM: the analytic harmonic trap, no first-derivative term;
+//   PH_ROT : the same plus the rotating-frame term om*i*(x D1y - y D1x);
+//   PH_GEN : pot read from a tabulated potential V (same padded layout as psi, one real
+//            per point; any of the reference's potential kinds), and a run-time drift:
+//            0 none, 1 the frame term, 2 the advective term 2 vx i D1x
+//            (src/finite_diff.f90:126-130).
+enum PhysKind { PH_HARM = 0, PH_ROT = 1, PH_GEN = 2 };
 enum StageKind { ST_RK1 = 0, ST_RK2 = 1, ST_RK3 = 2, ST_EULER = 3, ST_RHS = 4 };
 
 template <typename T> struct alignas(2 * sizeof(T)) cplx { T re, im; };
@@ -85,6 +93,9 @@ struct StageArgs {
   const double *c0dev;         // imaginary time: c0 from device memory (or null)
   const int *skip;             // imaginary time: converged flag; skip the pass when set
   double *partials;            // reductions: 4 doubles per workgroup
+  const void *vtab;            // PH_GEN: tabulated potential (padded layout, real T)
+  int drift;                   // PH_GEN: 0 none, 1 frame term (om), 2 advective (vx2)
+  double vx2;                  // PH_GEN: 2 * advective_vx
 };
 
 // ---- small helpers -------------------------------------------------------------
@@ -209,6 +220,34 @@ template <typename T, int VZ> struct BufIO {
   }
 };
 
+// VZ consecutive reals (the tabulated potential) at a byte offset of a plane resource.
+template <typename T, int VZ> struct BufIOR {
+  static constexpr int BYTES = VZ * (int)sizeof(T);
+  static_assert(BYTES == 4 || BYTES == 8 || BYTES == 16, "unsupported vector width");
+  static __device__ __forceinline__ void ld(__amdgpu_buffer_rsrc_t r, uint32_t off, T *o) {
+    if constexpr (BYTES == 4) {
+      o[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else if constexpr (BYTES == 8) {
+      const tbec_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+      if constexpr (sizeof(T) == 4) {
+        o[0] = __uint_as_float(v.x);
+        o[1] = __uint_as_float(v.y);
+      } else {
+        o[0] = __hiloint2double(v.y, v.x);
+      }
+    } else {
+      const tbec_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+      if constexpr (sizeof(T) == 4) {
+        o[0] = __uint_as_float(v.x); o[1] = __uint_as_float(v.y);
+        o[2] = __uint_as_float(v.z); o[3] = __uint_as_float(v.w);
+      } else {
+        o[0] = __hiloint2double(v.y, v.x);
+        o[1] = __hiloint2double(v.w, v.z);
+      }
+    }
+  }
+};
+
 // 7-tap second derivative sum (without the 1/(180 h^2) factor) of one component.
 template <typename T, int SUM>
 __device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
@@ -265,7 +304,7 @@ __device__ __forceinline__ void block_sum4(double v[4], double *out) {
 // ---- the stage kernel ----------------------------------------------------------
 // Process one tile (TZ x TY in z, y; one x segment) of one condensate: logical tile id
 // lb in [0, tiles_z*tiles_y*nseg).  Accumulates this thread's reduction terms in red.
-template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
 __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, double red[4]) {
   using C = cplx<T>;
   constexpr int NTZ = TZ / VZ;
@@ -366,6 +405,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // w of the first plane (stages that read w); later planes are prefetched one ahead
   C wq[2][VZ];
   if (stage_reads_w(a.stage)) BufIO<T, VZ>::ld(rs(p_w), ownb, wq[0]);
+  // tabulated potential (PH_GEN): own points of plane x, prefetched one plane ahead
+  constexpr bool VT = PHYS == PH_GEN;
+  const int vbytes = (int)(sx * (long long)sizeof(T));
+  const T *p_v = VT ? reinterpret_cast<const T *>(a.vtab) + rb + (long long)xs * sx : nullptr;
+  const uint32_t ownv = own * (uint32_t)sizeof(T);
+  T vq[2][VZ];
+  if constexpr (VT) BufIOR<T, VZ>::ld(plane_rsrc(p_v, vbytes), ownv, vq[0]);
+  const int drift = VT ? a.drift : 0;
+  const T vx2 = T(a.vx2);
 
   const T k2x = T(a.k2[0]), k2y = T(a.k2[1]), k2z = T(a.k2[2]);
   const T k1x = T(a.k1[0]), k1y = T(a.k1[1]);
@@ -400,6 +448,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     ldv(rs(p_own), ownb, q[(R + 7) & 7]);
     if (hl) ldv(rs(p_halo), hoffb, hb[buf ^ 1]);
     if (stage_reads_w(stage) && x + 1 < xe) BufIO<T, VZ>::ld(rs(p_wnext), ownb, wq[buf ^ 1]);
+    if constexpr (VT) {
+      if (x + 1 < xe) BufIOR<T, VZ>::ld(plane_rsrc(p_v + sx, vbytes), ownv, vq[buf ^ 1]);
+    }
     const C *wv = wq[buf];
 
     // stage centre plane + halo into LDS
@@ -447,7 +498,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         rr = (dxr * k2x + dyr * k2y) + dzr * k2z;
         ri = (dxi * k2x + dyi * k2y) + dzi * k2z;
       }
-      if constexpr (ROT) {
+      if (PHYS == PH_ROT || drift == 1) {
         const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
         const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
         const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
@@ -456,9 +507,16 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         const T pim = xc * d1yi - yc * d1xi;
         rr -= om * pim;   // (0, om) * (pr, pim)
         ri += om * pr;
+      } else if (VT && drift == 2) {
+        const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
+        const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
+        rr -= vx2 * d1xi;  // (0, 2 vx) * (d1xr, d1xi)
+        ri += vx2 * d1xr;
       }
       const T n2 = f.re * f.re + f.im * f.im;
-      const T pot = potxy + zc2[v];
+      T pot;
+      if constexpr (VT) pot = vq[buf][v];
+      else pot = potxy + zc2[v];
       const T fac = (c0 - gnl * n2) - pot;
       rr += fac * f.re;
       ri += fac * f.im;
@@ -521,6 +579,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     p_wnext += sx;
     p_out += sx;
     p_w += sx;
+    if constexpr (VT) p_v += sx;
   };
 
   using I0 = std::integral_constant<int, 0>;
@@ -552,7 +611,7 @@ template <typename T, int NT, bool IMAG> constexpr int stage_min_waves() {
   return (sizeof(T) == 4 && NT == 512) ? 4 : 1;
 }
 
-template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
 __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage(const StageArgs a) {
   constexpr int NT = (TZ / VZ) * TY;
@@ -566,7 +625,7 @@ gpe_stage(const StageArgs a) {
   int lb = b;
   if (a.remap && (nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  march_tile<T, TZ, TY, VZ, SUM, ROT, IMAG>(a, lb, red);
+  march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG>(a, lb, red);
   if constexpr (IMAG) {
     if (a.partials) block_sum4<NT>(red, a.partials + 4LL * lb);
   }
@@ -596,7 +655,7 @@ struct BatchLaunch {
   const int *flags;     // per-instance flags (stride 4); skip when flags[4*i] != 0
 };
 
-template <typename T, int TZ, int TY, int VZ, int SUM, bool ROT, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
 __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage_batch(const BatchLaunch L) {
   constexpr int NT = (TZ / VZ) * TY;
@@ -621,7 +680,7 @@ gpe_stage_batch(const BatchLaunch L) {
     a.c0dev = IMAG ? st + 1 : nullptr;                            // ST_C0EFF
     a.skip = nullptr;
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    march_tile<T, TZ, TY, VZ, SUM, ROT, IMAG>(a, item - bi.item_begin, red);
+    march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG>(a, item - bi.item_begin, red);
     if constexpr (IMAG) block_sum4<NT>(red, L.partials + 4LL * item);
     __syncthreads();
   }

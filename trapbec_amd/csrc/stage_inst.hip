@@ -17,9 +17,9 @@ namespace tbec {
 using InstT = std::conditional_t<INST_PREC == 32, float, double>;
 using InstShape = TileShape<INST_PREC, INST_TILE>;
 
-hipError_t TBEC_NAME(INST_PREC, INST_TILE)(const StageArgs &a, int sum, bool rot, bool imag,
+hipError_t TBEC_NAME(INST_PREC, INST_TILE)(const StageArgs &a, int sum, int phys, bool imag,
                                            hipStream_t s) {
-  return launch_tile<InstT, InstShape::TZ, InstShape::TY, InstShape::VZ>(a, sum, rot, imag, s);
+  return launch_tile<InstT, InstShape::TZ, InstShape::TY, InstShape::VZ>(a, sum, phys, imag, s);
 }
 
 hipError_t TBEC_BNAME(INST_PREC, INST_TILE)(const BatchLaunch &L, int sum, bool rot, int grid,

@@ -22,7 +22,7 @@ template <> struct TileShape<64, 3> { static constexpr int TZ = 64, TY = 16, VZ 
 template <> struct TileShape<64, 4> { static constexpr int TZ = 32, TY = 8, VZ = 1; };
 
 #define TBEC_DECL_LAUNCHER(P, I) \
-  hipError_t launch_stage_p##P##_t##I(const StageArgs &a, int sum, bool rot, bool imag, \
+  hipError_t launch_stage_p##P##_t##I(const StageArgs &a, int sum, int phys, bool imag, \
                                       hipStream_t s);
 TBEC_DECL_LAUNCHER(32, 0)
 TBEC_DECL_LAUNCHER(32, 1)
@@ -55,29 +55,31 @@ TBEC_DECL_BLAUNCHER(64, 4)
 hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool rot, int grid,
                         hipStream_t s);
 
-template <typename T, int TZ, int TY, int VZ, int S>
-static void launch_sum(const StageArgs &a, bool rot, bool imag, hipStream_t s) {
+template <typename T, int TZ, int TY, int VZ, int S, int PH>
+static void launch_phys(const StageArgs &a, bool imag, hipStream_t s) {
   constexpr int NT = (TZ / VZ) * TY;
   dim3 grid(a.nblocks), block(NT);
-  if (rot) {
-    if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, true, false>), grid, block, 0, s, a);
-  } else {
-    if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, false, false>), grid, block, 0, s, a);
-  }
+  if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false>), grid, block, 0, s, a);
+}
+
+template <typename T, int TZ, int TY, int VZ, int S>
+static void launch_sum(const StageArgs &a, int phys, bool imag, hipStream_t s) {
+  if (phys == PH_GEN) launch_phys<T, TZ, TY, VZ, S, PH_GEN>(a, imag, s);
+  else if (phys == PH_ROT) launch_phys<T, TZ, TY, VZ, S, PH_ROT>(a, imag, s);
+  else launch_phys<T, TZ, TY, VZ, S, PH_HARM>(a, imag, s);
 }
 
 // FP64 uses the reference summation only (it is the parity path); FP32 has all three.
 template <typename T, int TZ, int TY, int VZ>
-static hipError_t launch_tile(const StageArgs &a, int sum, bool rot, bool imag,
+static hipError_t launch_tile(const StageArgs &a, int sum, int phys, bool imag,
                               hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
-    launch_sum<T, TZ, TY, VZ, SUM_REF>(a, rot, imag, s);
+    launch_sum<T, TZ, TY, VZ, SUM_REF>(a, phys, imag, s);
   } else {
-    if (sum == SUM_REF) launch_sum<T, TZ, TY, VZ, SUM_REF>(a, rot, imag, s);
-    else if (sum == SUM_COMP) launch_sum<T, TZ, TY, VZ, SUM_COMP>(a, rot, imag, s);
-    else launch_sum<T, TZ, TY, VZ, SUM_CENTRED>(a, rot, imag, s);
+    if (sum == SUM_REF) launch_sum<T, TZ, TY, VZ, SUM_REF>(a, phys, imag, s);
+    else if (sum == SUM_COMP) launch_sum<T, TZ, TY, VZ, SUM_COMP>(a, phys, imag, s);
+    else launch_sum<T, TZ, TY, VZ, SUM_CENTRED>(a, phys, imag, s);
   }
   return hipGetLastError();
 }
@@ -87,11 +89,11 @@ template <typename T, int TZ, int TY, int VZ, int S>
 static void launch_batch_sum(const BatchLaunch &L, bool rot, int grid, hipStream_t s) {
   constexpr int NT = (TZ / VZ) * TY;
   if (rot)
-    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, true, true>), dim3(grid), dim3(NT), 0,
-                       s, L);
-  else
-    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, false, true>), dim3(grid), dim3(NT),
+    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH_ROT, true>), dim3(grid), dim3(NT),
                        0, s, L);
+  else
+    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH_HARM, true>), dim3(grid),
+                       dim3(NT), 0, s, L);
 }
 
 template <typename T, int TZ, int TY, int VZ>

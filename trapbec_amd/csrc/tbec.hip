@@ -74,23 +74,23 @@ int auto_seg(long long tiles, int nx, long long want) {
   return 8;
 }
 
-hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, bool rot,
+hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, int phys,
                         bool imag, hipStream_t s) {
   if (prec == TBEC_F32) {
     switch (tile) {
-      case 0: return launch_stage_p32_t0(a, sum, rot, imag, s);
-      case 1: return launch_stage_p32_t1(a, sum, rot, imag, s);
-      case 2: return launch_stage_p32_t2(a, sum, rot, imag, s);
-      case 3: return launch_stage_p32_t3(a, sum, rot, imag, s);
-      default: return launch_stage_p32_t4(a, sum, rot, imag, s);
+      case 0: return launch_stage_p32_t0(a, sum, phys, imag, s);
+      case 1: return launch_stage_p32_t1(a, sum, phys, imag, s);
+      case 2: return launch_stage_p32_t2(a, sum, phys, imag, s);
+      case 3: return launch_stage_p32_t3(a, sum, phys, imag, s);
+      default: return launch_stage_p32_t4(a, sum, phys, imag, s);
     }
   } else {
     switch (tile) {
-      case 0: return launch_stage_p64_t0(a, sum, rot, imag, s);
-      case 1: return launch_stage_p64_t1(a, sum, rot, imag, s);
-      case 2: return launch_stage_p64_t2(a, sum, rot, imag, s);
-      case 3: return launch_stage_p64_t3(a, sum, rot, imag, s);
-      default: return launch_stage_p64_t4(a, sum, rot, imag, s);
+      case 0: return launch_stage_p64_t0(a, sum, phys, imag, s);
+      case 1: return launch_stage_p64_t1(a, sum, phys, imag, s);
+      case 2: return launch_stage_p64_t2(a, sum, phys, imag, s);
+      case 3: return launch_stage_p64_t3(a, sum, phys, imag, s);
+      default: return launch_stage_p64_t4(a, sum, phys, imag, s);
     }
   }
 }
@@ -352,6 +352,7 @@ int fill_static_args(const tbec_params &p, const Layout &L, int tile, int seg, S
   a.remap = 1;
   a.scale = nullptr; a.prev_scale = nullptr; a.c0dev = nullptr; a.skip = nullptr;
   a.partials = nullptr;
+  a.vtab = nullptr; a.drift = 0; a.vx2 = 0.0;
   return a.nblocks;
 }
 
@@ -430,9 +431,17 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     a.w = a.psi_out;
     a.wrec = kind == ST_RK2 ? 3.0 : 1.0;
   }
+  // physics of the current step (tbec::prepare_physics)
+  if (h->ph_mode == PH_GEN) {
+    a.vtab = h->vtab;
+    a.drift = h->ph_drift;
+    a.vx2 = 2.0 * h->phys.advective_vx;
+  } else if (h->ph_wl_zero) {
+    a.wl[0] = a.wl[1] = a.wl[2] = 0.0;
+  }
   int rcg = refresh_ghosts(h);
   if (rcg) return rcg;
-  hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, handle_rot(h), want_red,
+  hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, h->ph_mode, want_red,
                               h->stream);
   if (e != hipSuccess)
     return fail(TBEC_ERR_HIP, "stage launch failed: %s", hipGetErrorString(e));
@@ -531,6 +540,7 @@ int tbec_destroy(tbec_handle *h) {
   if (!h) return 0;
   tbec_drop_graphs(h);
   if (h->stream) hipStreamSynchronize(h->stream);
+  free_physics(h);
   if (h->psi[0]) hipFree(h->psi[0]);
   if (h->psi[1]) hipFree(h->psi[1]);
   if (h->w) hipFree(h->w);
@@ -628,6 +638,7 @@ int tbec_download_psi_device(tbec_handle *h, void *psi_dev) {
 int tbec_rhs(tbec_handle *h, void *rhs_host) {
   if (!h || !rhs_host) return fail(TBEC_ERR_ARG, "null argument");
   int rc = tbec_materialize_scale(h);
+  if (!rc) rc = prepare_physics(h, h->t_now);
   if (rc) return rc;
   rc = enqueue_stage(h, ST_RHS, false, false);
   if (rc) return rc;
@@ -644,7 +655,8 @@ int tbec_rhs(tbec_handle *h, void *rhs_host) {
 int tbec_diagnose(tbec_handle *h, tbec_diag *out) {
   if (!h || !out) return fail(TBEC_ERR_ARG, "null argument");
   // rhs-only pass with reductions (psi read with the pending scale), then Psi_info
-  int rc = enqueue_stage(h, ST_RHS, false, true);
+  int rc = prepare_physics(h, h->t_now);
+  if (!rc) rc = enqueue_stage(h, ST_RHS, false, true);
   if (rc) return rc;
   const int nb_rhs = fill_args_count(h);
   hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <map>
 #include <string>
 #include <tuple>
@@ -84,6 +85,20 @@ struct tbec_handle {
   hipStream_t stream = nullptr;
   // captured step graphs: (mode, steps, parity) -> executable graph
   std::map<std::tuple<int, long long, int>, hipGraphExec_t> graphs;
+  // physics beyond the harmonic trap (tbec_set_physics)
+  tbec_physics phys{TBEC_POT_HARMONIC, 0, 4.0, HUGE_VAL, HUGE_VAL, 0.0};
+  double t_now = 0.0;                   // the handle's time t (real-time steps advance it)
+  void *vtab = nullptr;                 // tabulated potential: padded layout, one real per point
+  void *trap = nullptr;                 // Trap table (mobius or the caller's): interior layout
+  int trap_owner = 0;                   // kind whose Trap the table holds (MOBIUS / TABLE), 0 none
+  bool vtab_valid = false;
+  double vkey[5] = {0, 0, 0, 0, 0};     // time key of the table held in vtab
+  // stage configuration of the current step (set by tbec::prepare_physics)
+  int ph_mode = 0;                      // PH_HARM / PH_ROT / PH_GEN
+  int ph_drift = 0;                     // PH_GEN: 0 none, 1 frame term, 2 advective
+  bool ph_wl_zero = false;              // analytic modes: the potential is zero
+  int ph_sig = -1;                      // signature of the three above (graphs are dropped
+                                        // when it changes)
 };
 
 namespace tbec {
@@ -100,3 +115,15 @@ int refresh_ghosts(tbec_handle *h);   // ghost faces of the current psi (non-zer
 }  // namespace tbec
 void tbec_drop_graphs(tbec_handle *h);
 int fill_args_count(tbec_handle *h);
+namespace tbec {
+// Stage configuration and, for tabulated kinds, the potential table of a step that
+// starts at time t (enqueued on the handle's stream when the table changes).
+int prepare_physics(tbec_handle *h, double t);
+// Physics of the handle changes with t (switch-off times, time-dependent kinds).
+bool physics_time_dependent(const tbec_handle *h);
+// The step starting at t2 has the same stage configuration and table as one starting at t1.
+bool same_step_physics(const tbec_handle *h, double t1, double t2);
+// t + Re(dt) in the working precision (src/run.f90:55).
+double advance_time(const tbec_handle *h, double t);
+void free_physics(tbec_handle *h);
+}  // namespace tbec

@@ -112,13 +112,18 @@ int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout) {
   if (nsteps < 0) return set_error(TBEC_ERR_ARG, "nsteps must be >= 0");
   if (mode != TBEC_REAL_RK3 && mode != TBEC_IMAG_RK3 && mode != TBEC_IMAG_EULER_REF)
     return set_error(TBEC_ERR_ARG, "unknown step mode");
+  if (t_inout) {
+    int rc = tbec_set_time(h, *t_inout);
+    if (rc) return rc;
+  }
   if (mode == TBEC_IMAG_RK3) {
     if (h->p.dt_re != 0.0 || !(h->p.dt_im < 0.0))
       return set_error(TBEC_ERR_ARG, "TBEC_IMAG_RK3 needs dt = (0, -dtau) with dtau > 0");
     // a step-limited run: no convergence stop, flags cleared; scale, c0 and the step
     // counter continue from the previous imaginary-time call
     h->tol = 0.0;
-    int rc = 0;
+    int rc = prepare_physics(h, h->t_now);   // imaginary time does not advance t
+    if (rc) return rc;
     const double zero = 0.0;
     const int zflags[4] = {0, 0, 0, 0};
     HIPCHK2(hipMemcpyAsync(h->dstate + ST_TOL, &zero, sizeof(double), hipMemcpyHostToDevice,
@@ -128,21 +133,42 @@ int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout) {
     h->scale_pending = true;
     rc = run_steps(h, nsteps, mode, nsteps >= kChunk ? kChunk : 0);
     if (rc) return rc;
+    if (t_inout) *t_inout = h->t_now;
     return tbec_materialize_scale(h);
   }
   int rc = tbec_materialize_scale(h);
   if (rc) return rc;
-  rc = run_steps(h, nsteps, mode, nsteps >= kChunk ? kChunk : 0);
-  if (rc) return rc;
-  if (t_inout && mode == TBEC_REAL_RK3) {
-    // t = t + dt with the real part kept (src/run.f90:55), in the working precision
-    double t = *t_inout;
-    for (int64_t i = 0; i < nsteps; ++i) {
-      if (h->p.prec == TBEC_F32) t = (double)((float)t + (float)h->dt_re);
-      else t = t + h->dt_re;
-    }
-    *t_inout = t;
+  if (mode == TBEC_IMAG_EULER_REF) {
+    rc = prepare_physics(h, h->t_now);
+    if (!rc) rc = run_steps(h, nsteps, mode, nsteps >= kChunk ? kChunk : 0);
+    if (!rc && t_inout) *t_inout = h->t_now;
+    return rc;
   }
+  // Real time: each step sees the potential, frame term and switch-offs of the time it
+  // starts at; runs of steps whose physics agrees share one table and replay graphs.
+  double t = h->t_now;
+  const bool moves = physics_time_dependent(h);
+  long long done = 0;
+  while (done < nsteps) {
+    rc = prepare_physics(h, t);
+    if (rc) return rc;
+    long long n = 1;
+    double tn = advance_time(h, t);
+    if (!moves) {
+      for (; done + n < nsteps; ++n) tn = advance_time(h, tn);
+    } else {
+      while (done + n < nsteps && same_step_physics(h, t, tn)) {
+        ++n;
+        tn = advance_time(h, tn);
+      }
+    }
+    rc = run_steps(h, n, mode, n >= kChunk ? kChunk : 0);
+    if (rc) return rc;
+    done += n;
+    t = tn;
+  }
+  h->t_now = t;
+  if (t_inout) *t_inout = t;
   return 0;
 }
 
@@ -154,6 +180,7 @@ int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
   if (!(norm > 0) || max_steps < 1 || check_every < 1)
     return set_error(TBEC_ERR_ARG, "norm, max_steps and check_every must be positive");
   int rc = tbec_materialize_scale(h);
+  if (!rc) rc = prepare_physics(h, h->t_now);   // imaginary time does not advance t
   if (rc) return rc;
   h->norm_target = norm;
   h->tol = tol;
@@ -209,6 +236,7 @@ int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
 int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage) {
   if (!h || !ms_per_stage || nstages < 1) return set_error(TBEC_ERR_ARG, "bad argument");
   int rc = tbec_materialize_scale(h);
+  if (!rc) rc = prepare_physics(h, h->t_now);
   if (rc) return rc;
   // a handle with dt = (0, -dtau) times the imaginary-time stages (reductions and
   // finalisers included) from a fresh solver state with no convergence stop

@@ -10,6 +10,7 @@ there is no CPU fallback -- a missing library or GPU raises.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -57,6 +58,18 @@ class TbecIc(C.Structure):
                 ("n_lines", C.c_int32), ("seed", C.c_uint32)]
 
 
+class TbecPhysics(C.Structure):
+    _fields_ = [("potential", C.c_int32), ("lbeam", C.c_int32), ("wbeam", C.c_double),
+                ("potential_off", C.c_double), ("t_spindown", C.c_double),
+                ("advective_vx", C.c_double)]
+
+
+# the reference's `potential` names (src/finite_diff.f90:30-117) plus a caller's table
+POTENTIALS = {"zero": 0, "harmonic": 1, "harmonic_ring": 2, "harmonic_ring2": 3,
+              "barrier": 4, "box_trap": 5, "channel": 6, "quartic": 7, "sphere": 8,
+              "mobius": 9, "table": 10}
+
+
 class TbecError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -73,7 +86,9 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
            "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
-           "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries"]
+           "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries",
+           "tbec_set_physics", "tbec_set_potential_table", "tbec_potential", "tbec_set_time",
+           "tbec_imprint_vortex"]
 
 _lib = None
 
@@ -103,6 +118,11 @@ def load(path: str = LIB_PATH):
     L.tbec_set_w_storage.argtypes = [H, C.c_int]
     L.tbec_set_boundaries.argtypes = [H, C.c_int, C.c_int]
     L.tbec_init_state.argtypes = [H, C.POINTER(TbecIc)]
+    L.tbec_set_physics.argtypes = [H, C.POINTER(TbecPhysics)]
+    L.tbec_set_potential_table.argtypes = [H, vp]
+    L.tbec_potential.argtypes = [H, C.c_double, vp]
+    L.tbec_set_time.argtypes = [H, C.c_double]
+    L.tbec_imprint_vortex.argtypes = [H, C.c_double, C.c_double, C.c_double]
     L.tbec_sync.argtypes = [H]
     L.tbec_stream.argtypes = [H]
     L.tbec_stream.restype = C.c_void_p
@@ -267,6 +287,37 @@ class Condensate:
         (mirror about the edge point) or "periodic"; periodicz makes z periodic."""
         bc = BOUNDARIES[boundaries] if isinstance(boundaries, str) else int(boundaries)
         _check(load().tbec_set_boundaries(self._h, bc, int(bool(periodicz))))
+
+    def set_physics(self, potential="harmonic", lbeam: bool = False, wbeam: float = 4.0,
+                    potential_off=None, t_spindown=None, advective_vx: float = 0.0):
+        """The reference's potential kind (a name of POTENTIALS), laser beam, switch-off
+        times (None: never) and advective velocity (src/finite_diff.f90:30-130)."""
+        kind = POTENTIALS[potential] if isinstance(potential, str) else int(potential)
+        ph = TbecPhysics(kind, int(bool(lbeam)), wbeam,
+                         math.inf if potential_off is None else potential_off,
+                         math.inf if t_spindown is None else t_spindown, advective_vx)
+        _check(load().tbec_set_physics(self._h, C.byref(ph)))
+
+    def set_potential_table(self, trap: np.ndarray):
+        """Trap(k,j,i) for potential="table", indexed [x, y, z]."""
+        a = np.ascontiguousarray(trap, dtype=self._rdt())
+        assert a.shape == tuple(self.shape)
+        _check(load().tbec_set_potential_table(self._h, a.ctypes.data_as(C.c_void_p)))
+
+    def potential(self, t: float | None = None) -> np.ndarray:
+        """pot over the interior at time t (default: the condensate's t), [x, y, z]."""
+        out = np.empty(tuple(self.shape), dtype=self._rdt())
+        _check(load().tbec_potential(self._h, self.t if t is None else t,
+                                     out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_time(self, t: float):
+        _check(load().tbec_set_time(self._h, t))
+        self.t = t
+
+    def imprint_vortex(self, x0: float, y0: float, direction: float):
+        """force_random_points' vortex imprint at (x0, y0) with sense +1 or -1."""
+        _check(load().tbec_imprint_vortex(self._h, x0, y0, direction))
 
     def set_w_storage(self, stored: bool):
         """True: RK3 keeps w in its own array (the reference's scheme, 10 transfers per
