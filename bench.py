@@ -82,7 +82,7 @@ def time_steps(torch, c, steps, warmup):
 SUMS = {"reference": 0, "compensated": 1, "centred": 2}
 
 
-def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False):
+def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False, table=False):
     import torch
     from trapbec_amd import tbec, autotune
     sm = 0 if prec == 64 else SUMS[sum_name or args.fp32_sum]
@@ -96,12 +96,20 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False):
     c.set_tuning(tile, seg)
     c.set_xcd_remap(remap)
     c.init_state(spec)            # seeded state generated in HBM (not timed)
+    if table:
+        # the same harmonic trap, read from a tabulated potential instead of computed:
+        # the stage variant every non-harmonic kind uses (one real per point and stage)
+        tab = c.potential(0.0)
+        c.set_physics("table")
+        c.set_potential_table(tab)
+        del tab
     c.sync()
     ms, wall = time_steps(torch, c, steps, args.warmup)
     npts = pr.shape[0] * pr.shape[1] * pr.shape[2]
     gpts = npts / (ms * 1e-3) / 1e9
     achieved = npts * BYTES_PER_PT_STEP[prec] / (ms * 1e-3)
-    moved = npts * MOVED_PER_PT_STEP[(prec, w_stored)] / (ms * 1e-3)
+    moved_pp = MOVED_PER_PT_STEP[(prec, w_stored)] + (3 * prec // 8 if table else 0)
+    moved = npts * moved_pp / (ms * 1e-3)
     c.close()
     return {"ms_per_step": ms, "gpt_steps_per_s": gpts, "wall_s": wall,
             "achieved_Bps": achieved, "moved_Bps": moved, "tile": tile, "seg": seg,
@@ -222,6 +230,10 @@ def main():
             res["rot"] = v
             print(f"[bench] fp32 Omega=0.7: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
                   flush=True)
+            v = run_gpu(args, prec, steps, spec, pr, table=True)
+            res["table"] = v
+            print(f"[bench] fp32 tabulated potential: {v['ms_per_step']:.3f} ms/step",
+                  file=sys.stderr, flush=True)
         if prec == 32 and args.variants:
             for name in SUMS:
                 if name == args.fp32_sum:
@@ -276,6 +288,8 @@ def main():
         out[f"fp{prec}_stored_w"] = brief(v, args.steps if prec == 32 else args.fp64_steps)
     if "rot" in res:
         out["fp32_rotating_omega_0.7"] = brief(res["rot"], args.steps)
+    if "table" in res:
+        out["fp32_tabulated_potential"] = brief(res["table"], args.steps)
     for name, v in res.get("variants", {}).items():
         out.setdefault("fp32_variants", {})[name] = brief(v, args.steps)
     traffic = os.path.join(ROOT, "profiles", f"traffic_m{n}.json")

@@ -205,6 +205,8 @@ typedef struct {
   int32_t potential;        /* TBEC_POT_* */
   int32_t lbeam;            /* add the laser beam 20 exp(-|(x,y) - (70,0)|^2 / wbeam^2)
                                (src/finite_diff.f90:118-123) */
+  int32_t two_dim;          /* drop the z second derivative (two_dim, :25-27) */
+  int32_t pad_;
   double wbeam;             /* beam width */
   double potential_off;     /* pot = 0 while t > potential_off (:124); HUGE_VAL: never */
   double t_spindown;        /* frame term only while t < t_spindown (:128); HUGE_VAL: never */

@@ -119,7 +119,7 @@ def case_diag(name, prec, nprocs=1, shape=(16, 16, 16), box=(16.0, 16.0, 16.0), 
 
 def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), steps=20,
              t0=0.0, omega_z=0.0, potential_off=None, t_spindown=None, advective_vx=0.0,
-             lbeam=False, wbeam=4.0, harm=None, seed=17, constant_ic=False):
+             lbeam=False, wbeam=4.0, harm=None, seed=17, constant_ic=False, two_dim=False):
     """The potential select of get_rhs (src/finite_diff.f90:30-124) with the laser term,
     the switch-off times and the advective term: real-time RK3 from a seeded state
     restarted at t0 (so the time-dependent openings and tilts are exercised), or, for
@@ -130,7 +130,8 @@ def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), 
                        Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=steps,
                        omega_z=omega_z, potential=potential, potential_off=potential_off,
                        t_spindown=t_spindown, advective_vx=advective_vx, lbeam=lbeam,
-                       wbeam=wbeam, harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
+                       wbeam=wbeam, harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2],
+                       two_dim=two_dim)
     cdt = np.complex128 if prec == 64 else np.complex64
     rdt = np.float64 if prec == 64 else np.float32
     if constant_ic:
@@ -148,7 +149,7 @@ def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), 
                         periodicz=False, potential=potential,
                         potential_off=inf if potential_off is None else potential_off,
                         t_spindown=inf if t_spindown is None else t_spindown,
-                        advective_vx=advective_vx, lbeam=lbeam, wbeam=wbeam)
+                        advective_vx=advective_vx, lbeam=lbeam, wbeam=wbeam, two_dim=two_dim)
     return r
 
 
@@ -199,6 +200,12 @@ CASES = {
     "pot_harm_switch_f64": lambda: case_pot("pot_harm_switch_f64", 64, "harmonic",
                                             shape=(16, 16, 16), omega_z=-1.4,
                                             potential_off=0.0105, t_spindown=0.0155),
+    "pot_2d_f64": lambda: case_pot("pot_2d_f64", 64, "harmonic", shape=(32, 24, 1),
+                                   box=(16.0, 12.0, 1.0), harm=(256.0, 144.0, 0.0),
+                                   omega_z=-1.4, two_dim=True),
+    "pot_2d_f32": lambda: case_pot("pot_2d_f32", 32, "barrier", shape=(32, 24, 1),
+                                   box=(16.0, 12.0, 1.0), two_dim=True,
+                                   potential_off=0.0155),
     "pot_advective_f64": lambda: case_pot("pot_advective_f64", 64, "harmonic",
                                           shape=(16, 16, 16), omega_z=-1.4,
                                           advective_vx=0.5),

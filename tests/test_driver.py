@@ -175,3 +175,75 @@ def test_driver_solid_boundaries_match_reference(tmp_path):
     _, _, psi = _read_var(d / "var0001.dat", (16, 16, 16))
     ref = f["psi"]
     assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def _write_restart(d, psi0, t0, itime=1):
+    """var.dat in the reference's restart layout: [itime] [t] [Psi] (src/initial.f90:796-818)."""
+    d.mkdir(parents=True, exist_ok=True)
+    rdt = np.float64 if psi0.dtype == np.complex128 else np.float32
+    with open(d / "var.dat", "wb") as fh:
+        for payload in (np.array([itime], np.int32).tobytes(), np.array([t0], rdt).tobytes(),
+                        np.ascontiguousarray(psi0).tobytes()):
+            n = np.array([len(payload)], np.int32).tobytes()
+            fh.write(n + payload + n)
+
+
+def test_driver_rejects_unknown_potential(tmp_path):
+    _need_driver()
+    nml = tmp_path / "bad.nml"
+    _nml(nml, **dict(C1_16, potential="no_such_trap"))
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0
+    assert "wrong potential set" in (tmp_path / "ERROR").read_text()
+
+
+@pytest.mark.gpu
+def test_driver_box_trap_matches_reference(tmp_path):
+    """potential='box_trap' with the frame term spun down mid-run (t_spindown), restarted
+    at t = 15 through the namelist: the reference fixture's output (FP64, 1e-12)."""
+    _need_driver()
+    f = np.load(os.path.join(ROOT, "tests", "golden", "pot_box_f64.npz"))
+    shape = f["psi0"].shape
+    _write_restart(tmp_path / "data" / "proc000", f["psi0"], float(f["t0"]))
+    box = tuple(float(b) for b in f["box"])
+    h = tuple(float(x) for x in f["harm_w"])
+    nml = tmp_path / "run.nml"
+    _nml(nml, Nx=shape[0], Ny=shape[1], Nz=shape[2], Lx=box[0], Ly=box[1], Lz=box[2],
+         harm_wx=h[0], harm_wy=h[1], harm_wz=h[2], omega_z=float(f["omega_z"]),
+         init_cond="restart", potential="box_trap", t_spindown=float(f["t_spindown"]),
+         nsteps=20, shots=20, out_dir=str(tmp_path / "data"))
+    print(f"\n[case] driver box_trap {shape}", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, hdr, psi = _read_var(tmp_path / "data" / "proc000" / "var0001.dat", shape)
+    ref = f["psi"]
+    assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()
+    assert hdr[3] == pytest.approx(float(f["t"]), rel=1e-15)
+
+
+@pytest.mark.gpu
+def test_driver_forcing_matches_oracle(tmp_path):
+    """forcing with force_freq = 5 (src/run.f90:70-74): a vortex line is imprinted after
+    steps 5 and 10 at seeded positions in [-L/4, L/4]^2; the step-10 snapshot is written
+    before that step's imprint, as in the reference's loop.  Against the oracle stepping
+    and imprinting at the same draws (FP64, 1e-12)."""
+    _need_driver()
+    nml = tmp_path / "run.nml"
+    _nml(nml, **C1_16, nsteps=15, shots=10, forcing=True, force_freq=5,
+         out_dir=str(tmp_path / "data"))
+    print("\n[case] driver forcing 16^3", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, psi = _read_var(tmp_path / "data" / "proc000" / "var0001.dat", (16, 16, 16))
+    pr, psi0 = initial.config_c1(n=16)
+    s = oracle.Stepper(oracle.params(pr.shape, pr.box, pr.kernel_params()["harm_w"],
+                                     dt=1e-3), psi0)
+    seed = C1_16["seed"]
+    s.rk3(5)
+    u = initial.uniform(seed, 7, np.arange(3))
+    s.imprint_vortex(-4.0 + u[0] * 8.0, -4.0 + u[1] * 8.0, -1.0 if u[2] < 0.5 else 1.0)
+    s.rk3(5)
+    ref = s.interior
+    assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()

@@ -48,7 +48,8 @@ def _physics(f):
     return dict(potential=str(f["potential"]), lbeam=bool(f["lbeam"]),
                 wbeam=float(f["wbeam"]), potential_off=_opt(f, "potential_off", None),
                 t_spindown=_opt(f, "t_spindown", None),
-                advective_vx=float(f["advective_vx"]))
+                advective_vx=float(f["advective_vx"]),
+                two_dim=bool(f["two_dim"]) if "two_dim" in f else False)
 
 
 def _condensate(f):

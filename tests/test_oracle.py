@@ -32,7 +32,8 @@ def _stepper_from_fixture(f):
                       potential_off=opt("potential_off", None),
                       t_spindown=opt("t_spindown", None),
                       advective_vx=opt("advective_vx", 0.0),
-                      lbeam=opt("lbeam", False, bool), wbeam=opt("wbeam", 4.0))
+                      lbeam=opt("lbeam", False, bool), wbeam=opt("wbeam", 4.0),
+                      two_dim=opt("two_dim", False, bool))
     s = oracle.Stepper(p, f["psi0"], prec=prec, t=opt("t0", 0.0))
     if "potential" in f and str(f["potential"]) == "mobius":
         s.set_trap(s.mobius_trap())
@@ -56,7 +57,8 @@ def test_matches_reference_bitwise(name):
 
 POT = ["pot_ring_f64", "pot_ring2_f32", "pot_barrier_f64", "pot_laser_f64", "pot_box_f64",
        "pot_channel_f32", "pot_quartic_f32", "pot_quartic_f64", "pot_sphere_f64",
-       "pot_sphere_f32", "pot_mobius_f64", "pot_harm_switch_f64", "pot_advective_f64"]
+       "pot_sphere_f32", "pot_mobius_f64", "pot_harm_switch_f64", "pot_advective_f64",
+       "pot_2d_f64", "pot_2d_f32"]
 
 
 @pytest.mark.parametrize("name", POT)
@@ -65,8 +67,8 @@ def test_potentials_match_reference_bitwise(name):
     t = 50, the barrier switched off mid-run, the laser beam, the box trap's sin(0.1 t)
     tilt with the frame term spun down mid-run, the channel with the advective term,
     quartic (r**8), sphere, the mobius Trap table (src/initial.f90:160-196) and the
-    harmonic trap with both switch-off times: 20 (mobius 10) RK3 steps equal the
-    reference build bit for bit."""
+    harmonic trap with both switch-off times, two_dim on a 32 x 24 x 1 grid: 20 (mobius
+    10) RK3 steps equal the reference build bit for bit."""
     f = np.load(os.path.join(GOLDEN, name + ".npz"))
     s, prec = _stepper_from_fixture(f)
     s.rk3(int(f["steps"]))

@@ -431,7 +431,9 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     a.w = a.psi_out;
     a.wrec = kind == ST_RK2 ? 3.0 : 1.0;
   }
-  // physics of the current step (tbec::prepare_physics)
+  // physics of the current step (tbec::prepare_physics); two_dim drops D2z exactly
+  // (x + 0*y == x for finite y)
+  if (h->phys.two_dim) a.k2[2] = 0.0;
   if (h->ph_mode == PH_GEN) {
     a.vtab = h->vtab;
     a.drift = h->ph_drift;

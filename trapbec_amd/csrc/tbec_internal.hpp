@@ -86,7 +86,7 @@ struct tbec_handle {
   // captured step graphs: (mode, steps, parity) -> executable graph
   std::map<std::tuple<int, long long, int>, hipGraphExec_t> graphs;
   // physics beyond the harmonic trap (tbec_set_physics)
-  tbec_physics phys{TBEC_POT_HARMONIC, 0, 4.0, HUGE_VAL, HUGE_VAL, 0.0};
+  tbec_physics phys{TBEC_POT_HARMONIC, 0, 0, 0, 4.0, HUGE_VAL, HUGE_VAL, 0.0};
   double t_now = 0.0;                   // the handle's time t (real-time steps advance it)
   void *vtab = nullptr;                 // tabulated potential: padded layout, one real per point
   void *trap = nullptr;                 // Trap table (mobius or the caller's): interior layout

@@ -147,15 +147,13 @@ template <typename T> __device__ T pot_point(const PotArgs &a, int i, int j, int
   return pot;
 }
 
+// One x-row of z points per workgroup row: grid (ceil(nz/256), ny, nx), so the point's
+// indices come from the launch coordinates (no 64-bit division per point).
 template <typename T> __global__ void __launch_bounds__(256) k_potential(const PotArgs a, T *v) {
-  const long long n = (long long)a.nx * a.ny * a.nz;
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int k = (int)(q % a.nz);
-    const long long r = q / a.nz;
-    const int j = (int)(r % a.ny), i = (int)(r / a.ny);
-    v[a.base + i * a.sx + j * a.sy + k] = pot_point<T>(a, i, j, k);
-  }
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, i = blockIdx.z;
+  if (k >= a.nz) return;
+  v[a.base + i * a.sx + j * a.sy + k] = pot_point<T>(a, i, j, k);
 }
 
 // Pack a padded real table to interior order.
@@ -386,13 +384,14 @@ int eval_table(tbec_handle *h, double t, bool pot_on, void *table) {
     return set_error(TBEC_ERR_STATE, k == TBEC_POT_TABLE
                                          ? "TBEC_POT_TABLE needs tbec_set_potential_table first"
                                          : "the mobius Trap was replaced; select it again");
+  if (h->p.nx > 65535 || h->p.ny > 65535)
+    return set_error(TBEC_ERR_ARG, "tabulated potentials need nx, ny <= 65535");
   const PotArgs a = pot_args(h, t, pot_on);
-  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  const dim3 grid((unsigned)((h->p.nz + 255) / 256), (unsigned)h->p.ny, (unsigned)h->p.nx);
   if (is_f32(h))
-    hipLaunchKernelGGL(k_potential<float>, dim3(blocks_for(n)), dim3(256), 0, h->stream, a,
-                       (float *)table);
+    hipLaunchKernelGGL(k_potential<float>, grid, dim3(256), 0, h->stream, a, (float *)table);
   else
-    hipLaunchKernelGGL(k_potential<double>, dim3(blocks_for(n)), dim3(256), 0, h->stream, a,
+    hipLaunchKernelGGL(k_potential<double>, grid, dim3(256), 0, h->stream, a,
                        (double *)table);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, std::string("potential: ") +

@@ -13,6 +13,11 @@ module tbec_iface
                                 TBEC_SUM_CENTRED = 2
   integer(c_int), parameter :: TBEC_BC_ZERO = 0, TBEC_BC_SOLID = 1, TBEC_BC_PERIODIC = 2
   integer(c_int), parameter :: TBEC_REAL_RK3 = 0, TBEC_IMAG_RK3 = 1, TBEC_IMAG_EULER_REF = 2
+  ! the reference's `potential` names (src/finite_diff.f90:30-117), plus a caller's table
+  integer(c_int), parameter :: TBEC_POT_ZERO = 0, TBEC_POT_HARMONIC = 1, &
+    TBEC_POT_HARMONIC_RING = 2, TBEC_POT_HARMONIC_RING2 = 3, TBEC_POT_BARRIER = 4, &
+    TBEC_POT_BOX_TRAP = 5, TBEC_POT_CHANNEL = 6, TBEC_POT_QUARTIC = 7, TBEC_POT_SPHERE = 8, &
+    TBEC_POT_MOBIUS = 9, TBEC_POT_TABLE = 10
 
   type, bind(C) :: tbec_params
     integer(c_int32_t) :: nx, ny, nz
@@ -28,6 +33,11 @@ module tbec_iface
   type, bind(C) :: tbec_diag
     real(c_double) :: norm, energy, mu, residual
     real(c_double) :: max_abs, min_abs, mean_abs, int_psi2
+  end type
+
+  type, bind(C) :: tbec_physics
+    integer(c_int32_t) :: potential, lbeam, two_dim, pad_
+    real(c_double) :: wbeam, potential_off, t_spindown, advective_vx
   end type
 
   type, bind(C) :: tbec_solve_result
@@ -59,6 +69,22 @@ module tbec_iface
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: bc, periodicz
+    end function
+    integer(c_int) function tbec_set_physics(h, ph) bind(C, name="tbec_set_physics")
+      import :: c_ptr, c_int, tbec_physics
+      type(c_ptr), value :: h
+      type(tbec_physics), intent(in) :: ph
+    end function
+    integer(c_int) function tbec_set_potential_table(h, trap) &
+        bind(C, name="tbec_set_potential_table")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, trap
+    end function
+    integer(c_int) function tbec_imprint_vortex(h, x0, y0, dir) &
+        bind(C, name="tbec_imprint_vortex")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: x0, y0, dir
     end function
     integer(c_int) function tbec_velocity(h, vel) bind(C, name="tbec_velocity")
       import :: c_int, c_ptr

@@ -14,6 +14,7 @@
 program tbec_run
   use iso_c_binding
   use iso_fortran_env, only: real32, real64, int64
+  use, intrinsic :: ieee_arithmetic, only: ieee_value, ieee_positive_inf
   use tbec_iface
   use tbec_ic
   implicit none
@@ -38,6 +39,13 @@ program tbec_run
   logical :: imaginary_time = .false., ic_only = .false., full_binary_output = .true.
   logical :: vel_print = .false., qpress_print = .false., periodicz = .false.
   character(len=16) :: boundaries = 'zero'
+  ! physics of get_rhs (src/finite_diff.f90:30-130) and the forcing (src/run.f90:70-74)
+  character(len=32) :: potential = 'harmonic'
+  logical :: lbeam = .false., forcing = .false., two_dim = .false.
+  real(dp) :: wbeam = 4.0_dp, advective_vx = 0.0_dp
+  real(dp) :: potential_off = huge(1.0_dp), t_spindown = huge(1.0_dp)
+  integer :: force_freq = 10
+  character(len=256) :: trap_file = ''
   integer :: imag_nsteps = 0, imag_check = 50
   real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
   character(len=256) :: out_dir = 'data'
@@ -46,10 +54,12 @@ program tbec_run
     trap_omega, mu, g_phys, tf_norm, psi_scale, n_lines, line_x, line_y, line_q, xi, &
     ring_radius, seed, noise_amp, phase_noise_amp, imaginary_time, ic_only, &
     full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir, &
-    vel_print, qpress_print, boundaries, periodicz
+    vel_print, qpress_print, boundaries, periodicz, potential, lbeam, wbeam, &
+    potential_off, t_spindown, advective_vx, forcing, force_freq, trap_file, two_dim
 
   character(len=512) :: nml_file
   integer :: u, ios, itime, nstart, l
+  integer(int64) :: nforce = 0
   real(dp), allocatable :: xx(:), yy(:), zz(:)
   complex(dp), allocatable, target :: psi(:, :, :)
   complex(real32), allocatable, target :: psi4(:, :, :)
@@ -72,6 +82,8 @@ program tbec_run
   close(u)
   if (prec /= 32 .and. prec /= 64) call emergency_stop('prec must be 32 or 64')
   if (shots < 1) call emergency_stop('shots must be positive')
+  if (forcing .and. force_freq < 1) call emergency_stop('force_freq must be positive')
+  l = pot_kind()   ! a wrong name stops here, before the device is touched
   if (sum_mode < 0) then
     sum_mode = TBEC_SUM_REFERENCE
     if (prec == 32) sum_mode = TBEC_SUM_CENTRED
@@ -127,6 +139,7 @@ program tbec_run
   p%dt_re = dt_re; p%dt_im = dt_im; p%gamma = gamma
   call check(tbec_create(p, h))
   call set_bc(h)
+  call set_phys(h)
   call upload()
 
   if (imaginary_time .and. nstart == 1) then
@@ -138,6 +151,7 @@ program tbec_run
   itime = nstart - 1
   do while (itime < nsteps)
     l = min(shots - mod(itime, shots), nsteps - itime)
+    if (forcing) l = min(l, force_freq - mod(itime, force_freq))
     call check(tbec_step(h, int(l, c_int64_t), TBEC_REAL_RK3, t))
     itime = itime + l
     if (mod(itime, shots) == 0) then
@@ -147,6 +161,9 @@ program tbec_run
       call write_var(itime / shots)
       if (vel_print) call write_field('vel', itime / shots)
       if (qpress_print) call write_field('qpress', itime / shots)
+    end if
+    if (forcing) then
+      if (mod(itime, force_freq) == 0) call force_random_point()
     end if
   end do
   call check(tbec_sync(h))
@@ -203,6 +220,7 @@ contains
     if (dt_im == 0.0_dp) q%dt_im = -abs(dt_re)
     call check(tbec_create(q, hi))
     call set_bc(hi)
+    call set_phys(hi)
     if (prec == 64) then
       call check(tbec_upload_psi(hi, c_loc(psi)))
     else
@@ -253,6 +271,81 @@ contains
       write(uv) cmplx(psi, kind=real32)
     end if
     close(uv)
+  end subroutine
+
+  ! potential / lbeam / wbeam / potential_off / t_spindown / advective_vx as in
+  ! src/cparam.local; huge() switch-off times mean never
+  subroutine set_phys(hh)
+    type(c_ptr), intent(in) :: hh
+    type(tbec_physics) :: ph
+    real(dp), allocatable, target :: tab(:, :, :)
+    real(real32), allocatable, target :: tab4(:, :, :)
+    integer :: ut
+    ph%potential = pot_kind()
+    ph%lbeam = merge(1, 0, lbeam)
+    ph%two_dim = merge(1, 0, two_dim)
+    ph%pad_ = 0
+    ph%wbeam = wbeam
+    ph%potential_off = never_if_huge(potential_off)
+    ph%t_spindown = never_if_huge(t_spindown)
+    ph%advective_vx = advective_vx
+    call check(tbec_set_physics(hh, ph))
+    if (ph%potential /= TBEC_POT_TABLE) return
+    ! Trap(k,j,i) as raw reals of the working precision, z fastest
+    if (len_trim(trap_file) == 0) call emergency_stop('potential = table needs trap_file')
+    open(newunit=ut, file=trim(trap_file), access='stream', form='unformatted', &
+         status='old', action='read', iostat=ios)
+    if (ios /= 0) call emergency_stop('cannot open trap_file '//trim(trap_file))
+    if (prec == 64) then
+      allocate(tab(Nz, Ny, Nx))
+      read(ut, iostat=ios) tab
+      if (ios == 0) call check(tbec_set_potential_table(hh, c_loc(tab)))
+    else
+      allocate(tab4(Nz, Ny, Nx))
+      read(ut, iostat=ios) tab4
+      if (ios == 0) call check(tbec_set_potential_table(hh, c_loc(tab4)))
+    end if
+    close(ut)
+    if (ios /= 0) call emergency_stop('short trap_file '//trim(trap_file))
+  end subroutine
+
+  ! the reference's `potential` names (src/finite_diff.f90:30-117) plus 'table'
+  integer(c_int32_t) function pot_kind()
+    select case (trim(potential))
+    case ('zero'); pot_kind = TBEC_POT_ZERO
+    case ('harmonic'); pot_kind = TBEC_POT_HARMONIC
+    case ('harmonic_ring'); pot_kind = TBEC_POT_HARMONIC_RING
+    case ('harmonic_ring2'); pot_kind = TBEC_POT_HARMONIC_RING2
+    case ('barrier'); pot_kind = TBEC_POT_BARRIER
+    case ('box_trap'); pot_kind = TBEC_POT_BOX_TRAP
+    case ('channel'); pot_kind = TBEC_POT_CHANNEL
+    case ('quartic'); pot_kind = TBEC_POT_QUARTIC
+    case ('sphere'); pot_kind = TBEC_POT_SPHERE
+    case ('mobius'); pot_kind = TBEC_POT_MOBIUS
+    case ('table'); pot_kind = TBEC_POT_TABLE
+    case default
+      pot_kind = -1
+      call emergency_stop('wrong potential set')
+    end select
+  end function
+
+  real(c_double) function never_if_huge(x)
+    real(dp), intent(in) :: x
+    never_if_huge = x
+    if (x >= huge(1.0_dp)) never_if_huge = ieee_value(1.0_c_double, ieee_positive_inf)
+  end function
+
+  ! force_random_points (src/force.f90:12-44): a vortex line at a uniform random
+  ! position in [-L/4, L/4]^2 with a random sense, drawn here from the seeded
+  ! counter-based generator (the reference's generator is clock-seeded).
+  subroutine force_random_point()
+    real(dp) :: x0, y0, dir
+    x0 = -Lx / 4.0_dp + uniform(seed, 7_int64, 3 * nforce) * (Lx / 4.0_dp + Lx / 4.0_dp)
+    y0 = -Ly / 4.0_dp + uniform(seed, 7_int64, 3 * nforce + 1) * (Ly / 4.0_dp + Ly / 4.0_dp)
+    dir = 1.0_dp
+    if (uniform(seed, 7_int64, 3 * nforce + 2) < 0.5_dp) dir = -1.0_dp
+    nforce = nforce + 1
+    call check(tbec_imprint_vortex(h, x0, y0, dir))
   end subroutine
 
   ! boundaries / periodicz as in src/cparam.local, applied by ghost_comm's rules

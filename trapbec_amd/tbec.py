@@ -59,7 +59,8 @@ class TbecIc(C.Structure):
 
 
 class TbecPhysics(C.Structure):
-    _fields_ = [("potential", C.c_int32), ("lbeam", C.c_int32), ("wbeam", C.c_double),
+    _fields_ = [("potential", C.c_int32), ("lbeam", C.c_int32), ("two_dim", C.c_int32),
+                ("pad_", C.c_int32), ("wbeam", C.c_double),
                 ("potential_off", C.c_double), ("t_spindown", C.c_double),
                 ("advective_vx", C.c_double)]
 
@@ -289,11 +290,12 @@ class Condensate:
         _check(load().tbec_set_boundaries(self._h, bc, int(bool(periodicz))))
 
     def set_physics(self, potential="harmonic", lbeam: bool = False, wbeam: float = 4.0,
-                    potential_off=None, t_spindown=None, advective_vx: float = 0.0):
+                    potential_off=None, t_spindown=None, advective_vx: float = 0.0,
+                    two_dim: bool = False):
         """The reference's potential kind (a name of POTENTIALS), laser beam, switch-off
-        times (None: never) and advective velocity (src/finite_diff.f90:30-130)."""
+        times (None: never), advective velocity and two_dim (src/finite_diff.f90:23-130)."""
         kind = POTENTIALS[potential] if isinstance(potential, str) else int(potential)
-        ph = TbecPhysics(kind, int(bool(lbeam)), wbeam,
+        ph = TbecPhysics(kind, int(bool(lbeam)), int(bool(two_dim)), 0, wbeam,
                          math.inf if potential_off is None else potential_off,
                          math.inf if t_spindown is None else t_spindown, advective_vx)
         _check(load().tbec_set_physics(self._h, C.byref(ph)))
