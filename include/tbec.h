@@ -166,6 +166,44 @@ typedef struct {
 } tbec_ic;
 int tbec_init_state(tbec_handle *h, const tbec_ic *ic);
 
+/* ---- the reference's own initial states and insertions --------------------------
+ * The init_cond select of mesh_init (src/initial.f90:58-156, names in that order) and the
+ * insertions applied before the real-time loop (src/run.f90:26,37-40), evaluated on the
+ * device in the handle's precision, operand for operand as the reference writes them.
+ * Values the reference draws from its clock-seeded generator (random_phase's waves and
+ * amplitudes, random_points' centres, vortex_rings' angles and offsets, add_noise) are
+ * drawn here from the counter-based generator: stream 201 random_phase (5 draws per
+ * wave), 202 random_points (3 per line), 203 vortex_rings (6 per ring), 204 add_noise
+ * (2 per point at the flat interior index), each mapped as runif(a, b) = a + u*(b - a)
+ * in the working precision. */
+enum {
+  TBEC_IC_CONSTANT = 0, TBEC_IC_SOLITON = 1, TBEC_IC_RANDOM_PHASE = 2, TBEC_IC_CROW = 3,
+  TBEC_IC_SINGLE_VORTEX = 4, TBEC_IC_SAUSAGE = 5, TBEC_IC_RANDOM_POINTS = 6,
+  TBEC_IC_LEAP_FROG = 7, TBEC_IC_VORTEX_RINGS = 8, TBEC_IC_LINE_OF_LINES = 9,
+  TBEC_IC_LINKED_RINGS = 10, TBEC_IC_LATTICE = 11, TBEC_IC_LATTICE2 = 12, TBEC_IC_BUNDLE = 13
+};
+enum {
+  TBEC_INSERT_PCURRENT = 0,       /* setup_pcurrent (src/initial.f90:669-739), by the
+                                     handle's potential kind */
+  TBEC_INSERT_RING = 1,           /* setup_add_ring (:759-794) */
+  TBEC_INSERT_DIPOLE = 2,         /* setup_add_dipole (:839-870) */
+  TBEC_INSERT_PHASE_IMPRINT = 3,  /* phase_imprint (:820-836) */
+  TBEC_INSERT_NOISE = 4           /* add_noise (:741-757) */
+};
+typedef struct {
+  int32_t kind;             /* TBEC_IC_* (tbec_reference_state) */
+  int32_t winding_number;   /* single_vortex, phase_imprint */
+  int32_t vor_ring_count;   /* vortex_rings, 1..8 */
+  uint32_t seed;            /* drawn values, see above */
+  double ring_rad;          /* line_of_lines, linked_rings, vortex_rings, add_ring */
+} tbec_reference_ic;
+/* Replace psi by a reference state (harm_w of the params feeds the Thomas-Fermi ones;
+ * two_dim of tbec_set_physics drops random_phase's z wave). */
+int tbec_reference_state(tbec_handle *h, const tbec_reference_ic *ic);
+/* Apply one insertion (TBEC_INSERT_*) to psi; winding_number, ring_rad and seed are read
+ * from ic, the potential kind from tbec_set_physics. */
+int tbec_reference_insert(tbec_handle *h, int what, const tbec_reference_ic *ic);
+
 /* Set the norm target used by TBEC_IMAG_RK3 steps (default 1). */
 int tbec_set_norm(tbec_handle *h, double norm);
 

@@ -34,6 +34,12 @@ class OrcParams(C.Structure):
                 ("advective_vx", C.c_double), ("trap", C.c_void_p)]
 
 BOUNDARIES = {"zero": 0, "solid": 1, "periodic": 2}
+# the reference's init_cond names (src/initial.f90:58-156) and insertions (src/run.f90:26,37-40)
+IC_KINDS = {"constant": 0, "soliton": 1, "random_phase": 2, "crow": 3, "single_vortex": 4,
+            "sausage": 5, "random_points": 6, "leap_frog": 7, "vortex_rings": 8,
+            "line_of_lines": 9, "linked_rings": 10, "lattice": 11, "lattice2": 12,
+            "bundle": 13}
+INSERTS = {"pcurrent": 0, "ring": 1, "dipole": 2, "phase_imprint": 3, "noise": 4}
 # the reference's `potential` names (src/finite_diff.f90:30-117) plus a user table
 POTENTIALS = {"zero": 0, "harmonic": 1, "harmonic_ring": 2, "harmonic_ring2": 3,
               "barrier": 4, "box_trap": 5, "channel": 6, "quartic": 7, "sphere": 8,
@@ -70,6 +76,10 @@ def lib():
             getattr(L, "orc_mobius_trap" + suf).argtypes = [P, vp]
             getattr(L, "orc_imprint_vortex" + suf).argtypes = [P, vp, C.c_double,
                                                                 C.c_double, C.c_double]
+            getattr(L, "orc_reference_state" + suf).argtypes = [P, C.c_int, C.c_int,
+                                                                 C.c_double, vp, C.c_int, vp]
+            getattr(L, "orc_reference_insert" + suf).argtypes = [P, C.c_int, C.c_int,
+                                                                  C.c_double, vp, vp]
         L.orc_sums_f64.argtypes = [P, vp, vp, vp]
         L.orc_energy_f64.argtypes = [P, vp, vp, C.c_double, vp]
         L.orc_imag_rk3_step_f64.argtypes = [P, vp, vp, vp, C.c_double, vp, vp]
@@ -182,6 +192,22 @@ class Stepper:
         """force_random_points' imprint at a given centre and sense (src/force.f90:32-43)."""
         getattr(lib(), "orc_imprint_vortex" + self.suf)(C.byref(self.p), _ptr(self.psi),
                                                         x0, y0, direction)
+
+    def reference_state(self, kind: str, winding: int = 1, ring_rad: float = 4.0,
+                        prm=None) -> None:
+        """Replace psi by one of the reference's init_cond states; prm holds the drawn or
+        precomputed values the C function documents (working-precision values)."""
+        a = np.ascontiguousarray(np.zeros(1) if prm is None else prm, dtype=np.float64)
+        getattr(lib(), "orc_reference_state" + self.suf)(
+            C.byref(self.p), IC_KINDS[kind], winding, ring_rad, _ptr(a),
+            0 if prm is None else a.size, _ptr(self.psi))
+
+    def reference_insert(self, what: str, winding: int = 1, ring_rad: float = 4.0,
+                         prm=None) -> None:
+        """Apply one of the reference's insertions to psi."""
+        a = np.ascontiguousarray(np.zeros(1) if prm is None else prm, dtype=np.float64)
+        getattr(lib(), "orc_reference_insert" + self.suf)(
+            C.byref(self.p), INSERTS[what], winding, ring_rad, _ptr(a), _ptr(self.psi))
 
     # FP64-only quantities with no reference counterpart
     def energy(self) -> dict:

@@ -42,7 +42,8 @@ DEFAULTS = dict(
     shots=10, nsteps=10, boundaries="zero", periodicz=False, init_cond="constant",
     potential="harmonic", harm_wx=256.0, harm_wy=256.0, harm_wz=256.0,
     potential_off=None, gamma=0.0, two_dim=False, omega_z=0.0, advective_vx=0.0,
-    lbeam=False, wbeam=4.0, t_spindown=None,
+    lbeam=False, wbeam=4.0, t_spindown=None, winding_number=1, ring_rad=4.0,
+    add_ring=False, add_dipole=False, add_pcurrent=False, phase_imprint_on=False,
     imaginary_time=False, imag_nsteps=1, full_binary_output=True,
     imaginary_output=False, vel_print=False, qpress_print=False,
 )
@@ -83,18 +84,18 @@ def cparam_text(p: dict) -> str:
         f"real, parameter :: potential_off={off}",
         f"real, parameter :: gamma={_fl(p['gamma'])}",
         f"logical, parameter :: two_dim={_lg(p['two_dim'])}",
-        "logical, parameter :: phase_imprint_on=.false.",
+        f"logical, parameter :: phase_imprint_on={_lg(p['phase_imprint_on'])}",
         f"logical, parameter :: lbeam={_lg(p['lbeam'])}",
-        "logical, parameter :: add_pcurrent=.false.",
+        f"logical, parameter :: add_pcurrent={_lg(p['add_pcurrent'])}",
         f"real, parameter :: wbeam={_fl(p['wbeam'])}",
         f"real, parameter :: advective_vx={_fl(p['advective_vx'])}",
         f"real, parameter :: omega_z={_fl(p['omega_z'])}",
         f"real, parameter :: t_spindown={spin}",
-        "integer, parameter :: winding_number=1",
+        f"integer, parameter :: winding_number={int(p['winding_number'])}",
         "integer, parameter :: vor_ring_count=1",
-        "real,parameter :: ring_rad=4.",
-        "logical, parameter :: add_ring=.false.",
-        "logical, parameter :: add_dipole=.false.",
+        f"real,parameter :: ring_rad={_fl(p['ring_rad'])}",
+        f"logical, parameter :: add_ring={_lg(p['add_ring'])}",
+        f"logical, parameter :: add_dipole={_lg(p['add_dipole'])}",
         "integer, parameter :: vor_line_count=1",
         f"logical, parameter :: imaginary_time={_lg(p['imaginary_time'])}",
         f"integer, parameter :: imag_nsteps={p['imag_nsteps']}",

@@ -584,6 +584,329 @@ void FN(orc_qpressure)(const orc_params *p, T *psi_pad, T *qp_out) {
   free(lp);
 }
 
+
+/* ---- initial states and insertions (src/initial.f90:58-156,202-626,637-870) ------
+ * kind / what numbers follow the C ABI (TBEC_IC_*, TBEC_INSERT_*).  prm carries the
+ * values the reference draws at run time or forms once (T values held in doubles):
+ *   random_phase : 20 x (wave1, wave2, wave3, amp1, amp2)   (setup_quench's draws)
+ *   random_points: 20 x (x0, y0, dir)
+ *   vortex_rings : count x (c11 c12 c13 c21 c22 c23 c31 c32 c33, tx, ty, tz): the
+ *                  rotation coefficients as the reference groups them, and translations
+ *   bundle       : 21 x (p1, p2): the three bundles' centres
+ *   soliton      : (Re, Im) of exp(eye*pi)
+ * Integer powers are multiplied out left to right; x**1.15 calls pow. */
+
+static FN(cplx) FN(cmul)(FN(cplx) a, FN(cplx) b) {
+  FN(cplx) r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.re * b.im + a.im * b.re;
+  return r;
+}
+
+/* func*exp(s*eye*theta) of a straight line along the third axis at (d1, d2) from it:
+ * func = 1 - exp(-c*r**1.15), theta = atan2(d2, d1). */
+static FN(cplx) FN(line_factor)(T d1, T d2, T c, T s) {
+  const T r = SQRT(d1 * d1 + d2 * d2);
+  const T theta = ATAN2(d2, d1);
+  const T func = (T)1 - EXP(-(c * POW(r, (T)1.15)));
+  const T a = s * theta;
+  FN(cplx) f;
+  f.re = func * COS(a);
+  f.im = func * SIN(a);
+  return f;
+}
+
+/* ring_density (src/initial.f90:628-635) */
+static T FN(ring_density)(T r) {
+  const T c1 = (T)11 / (T)32;
+  const T c3 = ((T)5 - (T)32 * c1) / ((T)48 - (T)192 * c1);
+  const T c2 = c1 * (c3 - (T)0.25);
+  return SQRT(((r * r) * (c1 + c2 * (r * r))) / (((T)1 + c3 * (r * r)) + c2 * (r * r * r * r)));
+}
+
+/* ring_density(r)*exp(eye*theta) of a ring pair term: axial offset a, radial distance q
+ * from the ring axis, sign sg = +1 (q + R) or -1 (q - R). */
+static FN(cplx) FN(ring_term)(T a, T q, T R, int sg) {
+  const T qq = sg > 0 ? q + R : q - R;
+  const T theta = ATAN2(a, qq);
+  const T r = SQRT(a * a + qq * qq);
+  const T d = FN(ring_density)(r);
+  FN(cplx) f;
+  f.re = d * COS(theta);
+  f.im = d * SIN(theta);
+  return f;
+}
+
+static FN(cplx) FN(conj)(FN(cplx) a) { a.im = -a.im; return a; }
+
+/* setup_thomas_fermi's TF_dense before the 1/sqrt(2) (src/initial.f90:637-652) */
+static T FN(tf_dense)(const orc_params *p, T x, T y, T z) {
+  const T Lx = (T)p->Lx, Ly = (T)p->Ly, Lz = (T)p->Lz;
+  const T ax = x / Lx, ay = y / Ly, az = z / Lz;
+  return (((T)1 - (T)p->harm_w[0] * (ax * ax)) - (T)p->harm_w[1] * (ay * ay)) -
+         (T)p->harm_w[2] * (az * az);
+}
+
+static FN(cplx) FN(state_at)(const orc_params *p, int kind, int winding, T R,
+                             const double *prm, int nprm, T x, T y, T z) {
+  const T Lx = (T)p->Lx, Ly = (T)p->Ly, Lz = (T)p->Lz;
+  const T pi = (T)3.14159265359;
+  FN(cplx) psi = {(T)0, (T)0};
+  switch (kind) {
+    case 0: {                                          /* constant :59-66 */
+      const T c = (T)1 / SQRT((T)2);
+      psi.re = c; psi.im = c;
+      break;
+    }
+    case 1: {                                          /* soliton :67-80 */
+      T tf = FN(tf_dense)(p, x, y, z) / SQRT((T)2);
+      if (tf < (T)0) tf = (T)0;
+      const T ld = HYPOT(tf, tf);
+      if (x < (T)0) { psi.re = ld; psi.im = (T)0 * ld; }
+      else { psi.re = ld * (T)prm[0]; psi.im = ld * (T)prm[1]; }
+      break;
+    }
+    case 2: {                                          /* random_phase: setup_quench :346-375 */
+      for (int q = 0; q < 20; ++q) {
+        const double *w = prm + 5 * q;
+        T s = x * (T)w[0] + y * (T)w[1];
+        if (!p->two_dim) s = s + z * (T)w[2];
+        const T arg = (((s * (T)2) * pi) * Lx) / (Lx * Lx);   /* eye*s*2*pi/Lx */
+        FN(cplx) e = {COS(arg), SIN(arg)};
+        FN(cplx) amp = {(T)w[3], (T)w[4]};
+        const FN(cplx) t = FN(cmul)(amp, e);
+        psi.re = psi.re + t.re;
+        psi.im = psi.im + t.im;
+      }
+      T tf = FN(tf_dense)(p, x, y, z);                   /* multiply_thomas_fermi :654-667 */
+      if (tf < (T)0) tf = (T)0;
+      psi.re = psi.re * tf;
+      psi.im = psi.im * tf;
+      break;
+    }
+    case 3: {                                          /* crow :81-108 */
+      const T v2 = -Ly / (T)4;
+      const T v1 = (T)(-4) + COS((pi * z) / Lz);
+      psi = FN(line_factor)(x - v1, y - v2, (T)0.7, (T)1);
+      const T w1 = (T)4 - COS((pi * z) / Lz);
+      psi = FN(cmul)(psi, FN(line_factor)(x - w1, y - v2, (T)0.7, (T)(-1)));
+      break;
+    }
+    case 4:                                            /* single_vortex :109-121 */
+      psi = FN(line_factor)(x - (T)0, y - (T)0, (T)0.7, (T)winding);
+      break;
+    case 5: {                                          /* sausage :122-137 */
+      const T c = fabs((double)z) < 4.0 ? (T)0.1 : (T)0.7;
+      psi = FN(line_factor)(x - (T)0, y - (T)0, c, (T)1);
+      break;
+    }
+    case 6: {                                          /* random_points :268-302 */
+      psi.re = (T)0.7071; psi.im = (T)0.7071;
+      for (int q = 0; q < 20; ++q) {
+        const double *v = prm + 3 * q;
+        psi = FN(cmul)(psi, FN(line_factor)(x - (T)v[0], y - (T)v[1], (T)0.7, (T)v[2]));
+      }
+      break;
+    }
+    case 7: {                                          /* leap_frog :304-344 */
+      psi.re = (T)0.7071; psi.im = (T)0.7071;
+      const T sh = (T)3, s1 = (T)3, s2 = (T)12;
+      const T px[4] = {-sh, sh, -sh, sh}, py[4] = {s1, s2, -s1, -s2};
+      const T dir[4] = {(T)1, (T)1, (T)(-1), (T)(-1)};
+      for (int q = 0; q < 4; ++q)
+        psi = FN(cmul)(psi, FN(line_factor)(x - px[q], y - py[q], (T)0.7, dir[q]));
+      break;
+    }
+    case 8: {                                          /* vortex_rings :202-266 */
+      psi.re = (T)1; psi.im = (T)0;
+      for (int q = 0; q < nprm / 12; ++q) {
+        const double *c = prm + 12 * q;
+        const T tx = ((T)c[0] * x + (T)c[1] * y) + (T)c[2] * z;
+        const T ty = ((T)c[3] * x + (T)c[4] * y) + (T)c[5] * z;
+        const T tz = ((T)c[6] * x - (T)c[7] * y) + (T)c[8] * z;
+        const T dx = tx - (T)c[9], dy = ty - (T)c[10], dz = tz - (T)c[11];
+        const T qd = SQRT(dy * dy + dz * dz);
+        const T theta = ATAN2(dz, dy);
+        FN(cplx) p12[2];
+        for (int sgi = 0; sgi < 2; ++sgi) {
+          const T qq = sgi == 0 ? qd + R : qd - R;
+          const T r = SQRT(dx * dx + qq * qq);
+          const T d = SQRT(((r * r) * ((T)0.347 + (T)0.0286 * (r * r))) /
+                           (((T)1 + (T)0.3333 * (r * r)) + (T)0.0286 * (r * r * r * r)));
+          FN(cplx) e = {COS(theta), SIN(theta)};
+          FN(cplx) dd = {d, (T)0};
+          p12[sgi] = FN(cmul)(dd, e);
+        }
+        psi = FN(cmul)(psi, FN(cmul)(p12[0], FN(conj)(p12[1])));
+      }
+      break;
+    }
+    case 9: {                                          /* line_of_lines :541-585 */
+      const T c = (T)1 / SQRT((T)2);
+      psi.re = c; psi.im = c;
+      const T a = x - (T)0, qd = SQRT(y * y + z * z);
+      psi = FN(cmul)(psi, FN(ring_term)(a, qd, R, 1));
+      psi = FN(cmul)(psi, FN(conj)(FN(ring_term)(a, qd, R, -1)));
+      break;
+    }
+    case 10: {                                         /* linked_rings :587-626 */
+      const T t1 = R * (T)0;
+      const T a = x - (T)0, q1 = SQRT(y * y + (z - t1) * (z - t1));
+      psi = FN(cmul)(FN(ring_term)(a, q1, R, 1), FN(conj)(FN(ring_term)(a, q1, R, -1)));
+      const T t2 = R * (T)1.2;
+      const T b = y - (T)0, q2 = SQRT(x * x + (z - t2) * (z - t2));
+      psi = FN(cmul)(psi, FN(ring_term)(b, q2, R, 1));
+      psi = FN(cmul)(psi, FN(conj)(FN(ring_term)(b, q2, R, -1)));
+      break;
+    }
+    case 11:                                           /* lattice :377-420 (ramp = 0) */
+      psi.re = (T)1; psi.im = (T)1;
+      for (int v1 = 1; v1 <= 10; ++v1)
+        for (int v2 = 1; v2 <= 10; ++v2) {
+          const T p1 = (T)v1 * (Lx / (T)16) - Lx / (T)3;
+          const T p2 = (T)v2 * (Ly / (T)16) - Ly / (T)3;
+          psi = FN(cmul)(psi, FN(line_factor)(x - p1, y - p2, (T)0.7, (T)1));
+        }
+      break;
+    case 12: {                                         /* lattice2 :422-464 */
+      T tf = FN(tf_dense)(p, x, y, z) / SQRT((T)2);
+      if (tf < (T)0) tf = (T)0;
+      psi.re = tf; psi.im = tf;
+      for (int v1 = 1; v1 <= 3; ++v1)
+        for (int v2 = 1; v2 <= 3; ++v2) {
+          const T p1 = (T)v1 * (Lx / (T)4) - Lx / (T)2;
+          const T p2 = (T)v2 * (Ly / (T)4) - Ly / (T)2;
+          psi = FN(cmul)(psi, FN(line_factor)(x - p1, y - p2, (T)0.7, (T)1));
+        }
+      for (int v1 = 1; v1 <= 3; ++v1)
+        for (int v2 = 1; v2 <= 3; ++v2) {
+          const T p1 = ((T)v1 * (Lx / (T)4) + (T)10) - Lx / (T)2;
+          const T p2 = ((T)v2 * (Lz / (T)4) + (T)10) - Lz / (T)2;
+          psi = FN(cmul)(psi, FN(line_factor)(x - p1, z - p2, (T)0.7, (T)1));
+        }
+      break;
+    }
+    case 13: {                                         /* bundle :466-539 */
+      T tf = FN(tf_dense)(p, x, y, z) / SQRT((T)2);
+      if (tf < (T)0) tf = (T)0;
+      psi.re = tf; psi.im = tf;
+      for (int q = 0; q < 7; ++q)
+        psi = FN(cmul)(psi, FN(line_factor)(x - (T)prm[2 * q], y - (T)prm[2 * q + 1],
+                                            (T)0.7, (T)1));
+      for (int q = 7; q < 14; ++q)
+        psi = FN(cmul)(psi, FN(line_factor)(x - (T)prm[2 * q], z - (T)prm[2 * q + 1],
+                                            (T)0.7, (T)1));
+      for (int q = 14; q < 21; ++q)
+        psi = FN(cmul)(psi, FN(line_factor)(y - (T)prm[2 * q], z - (T)prm[2 * q + 1],
+                                            (T)0.7, (T)1));
+      break;
+    }
+    default:
+      break;
+  }
+  return psi;
+}
+
+/* One of the reference's init_cond states over this block's interior. */
+void FN(orc_reference_state)(const orc_params *p, int kind, int winding, double ring_rad,
+                             const double *prm, int nprm, T *psi_pad) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6);
+  T *xx = malloc(sizeof(T) * mx), *yy = malloc(sizeof(T) * my), *zz = malloc(sizeof(T) * mz);
+  FN(coords)(p, xx, yy, zz);
+  FN(cplx) *P = (FN(cplx) *)psi_pad;
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= mx; ++i)
+    for (int j = 1; j <= my; ++j)
+      for (int k = 1; k <= mz; ++k)
+        P[(long)(i + 2) * sx + (long)(j + 2) * sy + (k + 2)] =
+            FN(state_at)(p, kind, winding, (T)ring_rad, prm, nprm, xx[i - 1], yy[j - 1],
+                         zz[k - 1]);
+  free(xx);
+  free(yy);
+  free(zz);
+}
+
+/* The insertions applied to an existing state (src/run.f90:26,37-40): what 0
+ * setup_pcurrent (:669-739, by potential kind), 1 setup_add_ring (:759-794, option 1),
+ * 2 setup_add_dipole (:839-870), 3 phase_imprint (:820-836), 4 add_noise (:741-757, prm
+ * holds 2 uniform draws per point in [0, 1), point-major). */
+void FN(orc_reference_insert)(const orc_params *p, int what, int winding, double ring_rad,
+                              const double *prm, T *psi_pad) {
+  const int mx = p->nx / p->dims_x, my = p->ny / p->dims_y, mz = p->nz;
+  const long sy = mz + 6, sx = (long)(my + 6) * (mz + 6);
+  T *xx = malloc(sizeof(T) * mx), *yy = malloc(sizeof(T) * my), *zz = malloc(sizeof(T) * mz);
+  FN(coords)(p, xx, yy, zz);
+  FN(cplx) *P = (FN(cplx) *)psi_pad;
+  const T Lx = (T)p->Lx, Ly = (T)p->Ly, R = (T)ring_rad;
+  const T pi = (T)3.14159265359;
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= mx; ++i)
+    for (int j = 1; j <= my; ++j)
+      for (int k = 1; k <= mz; ++k) {
+        FN(cplx) *c = P + (long)(i + 2) * sx + (long)(j + 2) * sy + (k + 2);
+        const T x = xx[i - 1], y = yy[j - 1], z = zz[k - 1];
+        if (what == 0 || what == 3) {
+          T theta = (T)0, n = (T)1;
+          int set = 1;
+          if (what == 3) {
+            theta = ATAN2(y - (T)0, x - (T)0);
+            n = (T)winding;
+          } else {
+            switch (p->potential) {
+              case ORC_POT_BARRIER:
+                theta = y < (T)0 ? ((x * (T)50) * pi) / Lx
+                                 : (T)20 * pi - ((x * (T)50) * pi) / Lx;
+                break;
+              case ORC_POT_MOBIUS:
+                theta = (T)20 * ATAN2(y, x);
+                break;
+              case ORC_POT_HARMONIC_RING: {
+                const T ax = x / Lx, ay = y / Lx;
+                const T r = SQRT(ax * ax + ay * ay);
+                theta = r < (T)0.3 ? ATAN2(y, x) : ATAN2(-y, x);
+                n = (T)10;
+                break;
+              }
+              case ORC_POT_HARMONIC_RING2:
+                theta = y < (T)0 ? ATAN2((T)0.5 * (y + Ly / (T)5), x)
+                                 : ATAN2((T)0.5 * (y - Ly / (T)5), x) - pi;
+                n = (T)20;
+                break;
+              case ORC_POT_CHANNEL:
+                theta = (((T)20 * pi) * (x + (T)0.5 * Lx)) / Lx;
+                n = (T)2;
+                break;
+              default:
+                set = 0;
+            }
+          }
+          if (set) {
+            const T ld = HYPOT(c->re, c->im);
+            const T a = n * theta;
+            c->re = ld * COS(a);
+            c->im = ld * SIN(a);
+          }
+        } else if (what == 1) {
+          const T tx = (T)(-0.25) * Lx;
+          const T a = x - tx, qd = SQRT(y * y + (z - (T)0) * (z - (T)0));
+          FN(cplx) v = FN(cmul)(*c, FN(ring_term)(a, qd, R, 1));
+          *c = FN(cmul)(v, FN(conj)(FN(ring_term)(a, qd, R, -1)));
+        } else if (what == 2) {
+          *c = FN(cmul)(*c, FN(line_factor)(x - (T)(-20), y - (T)5, (T)0.7, (T)1));
+          *c = FN(cmul)(*c, FN(line_factor)(x - (T)(-20), y - (T)(-5), (T)0.7, (T)(-1)));
+        } else if (what == 4) {
+          const long q = ((long)(i - 1) * my + (j - 1)) * mz + (k - 1);
+          const T e1 = (T)prm[2 * q] * (T)2 - (T)1, e2 = (T)prm[2 * q + 1] * (T)2 - (T)1;
+          c->re = c->re + (T)0.01 * e1;
+          c->im = c->im + (T)0.01 * e2;
+        }
+      }
+  free(xx);
+  free(yy);
+  free(zz);
+}
+
 #undef FN
 #undef CAT
 #undef CAT_

@@ -153,6 +153,64 @@ def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), 
     return r
 
 
+def case_ic(name, prec, init_cond, shape=(20, 16, 12), box=(16.0, 14.0, 12.0),
+            harm=(4.0, 4.0, 4.0), potential="harmonic", winding_number=1, ring_rad=4.0,
+            add_ring=False, add_dipole=False, add_pcurrent=False, phase_imprint_on=False):
+    """One of the reference's own initial states (src/initial.f90:58-156) with the
+    insertions of src/run.f90:26,37-40, read back from var0001.dat after one step with
+    dt = 0 (which leaves psi unchanged: w = i*0.5*0*rhs = 0).  phase_imprint runs in the
+    reference's imaginary stage, one Euler step with the same dt = 0."""
+    d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                       Lx=box[0], Ly=box[1], Lz=box[2], nsteps=1, shots=1, dt_re=0.0,
+                       dt_im=0.0, init_cond=init_cond, potential=potential,
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2],
+                       winding_number=winding_number, ring_rad=ring_rad, add_ring=add_ring,
+                       add_dipole=add_dipole, add_pcurrent=add_pcurrent,
+                       phase_imprint_on=phase_imprint_on, imaginary_time=phase_imprint_on,
+                       imag_nsteps=1)
+    r = refbuild.run(d, None, prec=prec, nprocs=1, shape=shape)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi=r["psi"], prec=prec,
+                        box=np.array(box), harm_w=np.array(harm), init_cond=init_cond,
+                        potential=potential, winding_number=winding_number,
+                        ring_rad=ring_rad, add_ring=add_ring, add_dipole=add_dipole,
+                        add_pcurrent=add_pcurrent, phase_imprint_on=phase_imprint_on)
+    return r
+
+
+IC_CASES = {
+    "ic_constant_f64": dict(prec=64, init_cond="constant"),
+    "ic_soliton_f64": dict(prec=64, init_cond="soliton"),
+    "ic_crow_f64": dict(prec=64, init_cond="crow", box=(16.0, 16.0, 12.0)),
+    "ic_crow_f32": dict(prec=32, init_cond="crow", box=(16.0, 16.0, 12.0)),
+    "ic_single_vortex_f64": dict(prec=64, init_cond="single_vortex", winding_number=2),
+    "ic_sausage_f64": dict(prec=64, init_cond="sausage"),
+    "ic_leap_frog_f64": dict(prec=64, init_cond="leap_frog", box=(16.0, 32.0, 12.0)),
+    "ic_line_of_lines_f64": dict(prec=64, init_cond="line_of_lines", ring_rad=3.0),
+    "ic_linked_rings_f64": dict(prec=64, init_cond="linked_rings", ring_rad=3.0),
+    "ic_lattice_f64": dict(prec=64, init_cond="lattice"),
+    "ic_lattice_f32": dict(prec=32, init_cond="lattice"),
+    "ic_lattice2_f64": dict(prec=64, init_cond="lattice2", box=(40.0, 40.0, 40.0),
+                            harm=(1.0, 1.0, 1.0)),
+    "ic_bundle_f64": dict(prec=64, init_cond="bundle", box=(80.0, 80.0, 80.0),
+                          harm=(1.0, 1.0, 1.0)),
+    "ins_ring_f64": dict(prec=64, init_cond="constant", add_ring=True, ring_rad=3.0),
+    "ins_dipole_f64": dict(prec=64, init_cond="constant", add_dipole=True,
+                           box=(48.0, 24.0, 12.0)),
+    "ins_pcurrent_barrier_f64": dict(prec=64, init_cond="constant", add_pcurrent=True,
+                                     potential="barrier"),
+    "ins_pcurrent_mobius_f64": dict(prec=64, init_cond="constant", add_pcurrent=True,
+                                    potential="mobius"),
+    "ins_pcurrent_ring_f64": dict(prec=64, init_cond="constant", add_pcurrent=True,
+                                  potential="harmonic_ring"),
+    "ins_pcurrent_ring2_f64": dict(prec=64, init_cond="constant", add_pcurrent=True,
+                                   potential="harmonic_ring2"),
+    "ins_pcurrent_channel_f32": dict(prec=32, init_cond="constant", add_pcurrent=True,
+                                     potential="channel"),
+    "ins_phase_imprint_f64": dict(prec=64, init_cond="constant", phase_imprint_on=True,
+                                  winding_number=2),
+}
+
+
 CASES = {
     "rk3_f64_16": lambda: case_small("rk3_f64_16", 64),
     "rk3_f64_16_rot": lambda: case_small("rk3_f64_16_rot", 64, omega_z=-1.4),
@@ -210,6 +268,9 @@ CASES = {
                                           shape=(16, 16, 16), omega_z=-1.4,
                                           advective_vx=0.5),
 }
+
+for _n, _kw in IC_CASES.items():
+    CASES[_n] = (lambda n, kw: (lambda: case_ic(n, **kw)))(_n, _kw)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

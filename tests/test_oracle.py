@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, oracle
+import refic_draws
 from trapbec_amd import initial
 
 SMALL = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "rk3_*.npz")))
@@ -468,3 +469,140 @@ def test_imprint_vortex_closed_forms():
     s2 = oracle.Stepper(p, psi0)
     s2.imprint_vortex(x0, y0, -1.0)
     assert set(s2.vortices(1e-6, 0)[:, 3]) == {-1}
+
+
+IC_FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "ic_*.npz")))
+INS_FIXTURES = sorted(os.path.basename(p)[:-4]
+                      for p in glob.glob(os.path.join(GOLDEN, "ins_*.npz")))
+
+
+def oracle_ic(f):
+    """The oracle's restatement of a fixture's initial state and insertions."""
+    prec = int(f["prec"])
+    box = tuple(float(b) for b in f["box"])
+    shape = f["psi"].shape
+    kind = str(f["init_cond"])
+    p = oracle.params(shape, box, tuple(float(h) for h in f["harm_w"]),
+                      potential=str(f["potential"]))
+    s = oracle.Stepper(p, np.zeros(shape, f["psi"].dtype), prec=prec)
+    wn, rr = int(f["winding_number"]), float(f["ring_rad"])
+    prm = None
+    if kind == "bundle":
+        prm = refic_draws.bundle_centres(prec)
+    elif kind == "soliton":
+        prm = refic_draws.exp_i_pi(prec)
+    s.reference_state(kind, wn, rr, prm)
+    if bool(f["phase_imprint_on"]):
+        s.reference_insert("phase_imprint", wn, rr)
+    if bool(f["add_pcurrent"]):
+        if str(f["potential"]) == "mobius":
+            s.set_trap(s.mobius_trap())
+        s.reference_insert("pcurrent", wn, rr)
+    if bool(f["add_ring"]):
+        s.reference_insert("ring", wn, rr)
+    if bool(f["add_dipole"]):
+        s.reference_insert("dipole", wn, rr)
+    return s.interior
+
+
+@pytest.mark.parametrize("name", IC_FIXTURES + INS_FIXTURES)
+def test_initial_states_match_reference(name):
+    """The reference's own init_cond states (constant, soliton, crow, single_vortex with
+    winding 2, sausage, leap_frog, line_of_lines, linked_rings, lattice, lattice2,
+    bundle) and insertions (add_ring, add_dipole, add_pcurrent for each potential that
+    defines one, phase_imprint), from var0001.dat of a dt = 0 run: equal as values, except
+    the soliton's exp(eye*pi), which amdflang folds at compile time (rounding).  Values,
+    not bytes: amdflang multiplies real x complex as the full complex product
+    (d, 0)*(c, s), whose d*s + 0*c turns a -0 into +0 where the oracle keeps d*s; such
+    signed zeros (3 points of the bundle state, in its zero-density region) compare
+    equal and enter no later arithmetic."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    got, ref = oracle_ic(f), f["psi"]
+    assert got.dtype == ref.dtype
+    if str(f["init_cond"]) == "soliton":
+        assert np.abs(got - ref).max() <= 4 * np.finfo(ref.real.dtype).eps * np.abs(ref).max()
+        return
+    assert np.array_equal(got, ref), \
+        f"{np.sum(got != ref)} points differ, max {np.abs(got - ref).max()}"
+
+
+def test_random_phase_is_the_drawn_plane_waves():
+    """setup_quench (src/initial.f90:346-375) with a flat Thomas-Fermi factor (harm_w = 0):
+    the state is the sum of the 20 drawn plane waves exp(2 pi i k.x / L), so its discrete
+    Fourier transform on the cell-centred 40^3 grid holds each drawn amplitude at its wave
+    vector (duplicates add) and nothing else."""
+    n, L = 40, 10.0
+    prm = refic_draws.random_phase(7, 64)
+    s = oracle.Stepper(oracle.params((n, n, n), L, (0.0, 0.0, 0.0)),
+                       np.zeros((n, n, n), np.complex128))
+    s.reference_state("random_phase", prm=prm)
+    F = np.fft.fftn(s.interior) / n ** 3
+    x0 = (0.5 / n - 0.5) * L          # the first cell centre
+    want = np.zeros_like(F)
+    for q in range(20):
+        k = prm[5 * q:5 * q + 3].astype(int)
+        amp = prm[5 * q + 3] + 1j * prm[5 * q + 4]
+        want[tuple(k % n)] += amp * np.exp(2j * np.pi * k.sum() * x0 / L)
+    assert np.abs(F - want).max() < 1e-12
+
+
+def test_random_points_and_dipole_windings():
+    """setup_random_points' 20 drawn lines (all positive, as the reference's draw in
+    [0.9, 1] makes them) and setup_add_dipole's pair: the plaquette windings of every
+    z-plane sum to the lines' charges, each found within one cell of a drawn centre."""
+    shape, box = (64, 64, 4), (24.0, 24.0, 4.0)
+    prm = refic_draws.random_points(11, box[0], box[1], 64)
+    s = oracle.Stepper(oracle.params(shape, box, (1.0, 1.0, 1.0)),
+                       np.zeros(shape, np.complex128))
+    s.reference_state("random_points", prm=prm)
+    v = s.vortices(1e-12, 0)
+    h = box[0] / shape[0]
+    cx, cy = (v[:, 0] + 1) * h - box[0] / 2, (v[:, 1] + 1) * h - box[1] / 2
+    centres = prm.reshape(20, 3)[:, :2]
+    d = np.hypot(cx[:, None] - centres[None, :, 0], cy[:, None] - centres[None, :, 1])
+    assert np.all(d.min(axis=1) < h)
+    for z in range(shape[2]):
+        assert v[v[:, 2] == z, 3].sum() == 20
+    shape, box = (48, 32, 2), (48.0, 24.0, 2.0)
+    s = oracle.Stepper(oracle.params(shape, box, (1.0, 1.0, 1.0)),
+                       np.full(shape, 1.0 + 0j))
+    s.reference_insert("dipole")
+    v = s.vortices(1e-12, 0)
+    assert sorted(v[v[:, 2] == 0, 3]) == [-1, 1]
+
+
+def test_vortex_ring_geometry():
+    """setup_vortex_rings (src/initial.f90:202-266) with one drawn ring.  Both factors
+    take the same azimuth theta = atan2(tz, ty), so ring_p1*conjg(ring_p2) is real: the
+    reference's "ring" is a density dip with no circulation (imaginary part at rounding
+    level).  The dip's zeros, d(r2) = 0, lie on the circle of radius ring_rad about the
+    drawn centre, perpendicular to the rotated x axis: the 40 lowest-density points are
+    within a cell of it."""
+    n, L, R = 48, 16.0, 3.0
+    prm = refic_draws.vortex_rings(5, 1, L, 64)
+    s = oracle.Stepper(oracle.params((n, n, n), L, (1.0, 1.0, 1.0)),
+                       np.zeros((n, n, n), np.complex128))
+    s.reference_state("vortex_rings", ring_rad=R, prm=prm)
+    psi = s.interior
+    assert np.abs(psi.imag).max() <= 1e-15 * np.abs(psi).max()
+    assert psi.real.min() >= 0
+    X, Y, Z = initial.mesh((n, n, n), (L, L, L))
+    low = np.argsort(np.abs(psi).reshape(-1))[:40]
+    pts = np.stack([X.reshape(-1)[low], Y.reshape(-1)[low], Z.reshape(-1)[low]])
+    c = prm[:9].reshape(3, 3)
+    rot = np.array([c[0], c[1], [c[2][0], -c[2][1], c[2][2]]])   # temp = rot @ x
+    t = rot @ pts - prm[9:12, None]
+    h = L / n
+    assert np.all(np.abs(t[0]) < h) and np.all(np.abs(np.hypot(t[1], t[2]) - R) < h)
+
+
+def test_add_noise_range():
+    """add_noise (src/initial.f90:741-757): psi += 0.01*(2u-1, 2u'-1) per point."""
+    shape = (8, 6, 4)
+    n = 8 * 6 * 4
+    u = refic_draws.noise(3, n)
+    s = oracle.Stepper(oracle.params(shape, 8.0, (1.0, 1.0, 1.0)), np.zeros(shape, np.complex128))
+    s.reference_insert("noise", prm=u)
+    d = s.interior.reshape(-1)
+    assert np.abs(d.real).max() <= 0.01 and np.abs(d.imag).max() <= 0.01
+    assert np.allclose(d.real, 0.01 * (2 * u[0::2] - 1), rtol=0, atol=1e-17)

@@ -18,20 +18,6 @@ using namespace tbec;
 
 namespace {
 
-__device__ __forceinline__ uint32_t hash32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x21F0AAADu;
-  x ^= x >> 15;
-  x *= 0x735A2D97u;
-  x ^= x >> 15;
-  return x;
-}
-
-__device__ __forceinline__ double uniform01(uint32_t seed, uint32_t stream, uint32_t idx) {
-  const uint32_t key = hash32(seed ^ hash32(stream));
-  return ((double)hash32(key ^ hash32(idx)) + 0.5) / 4294967296.0;
-}
-
 struct IcArgs {
   tbec_ic ic;
   int nx, ny, nz;

@@ -50,6 +50,22 @@ long long tiles_per_plane(const tbec_params &p, int tile);
 int auto_seg(long long tiles, int nx, long long want);
 constexpr long long kAutoSegBlocks = 1024;
 
+// Counter-based random numbers: a function of (seed, stream, index) only, identical on
+// the host, the device and in trapbec_amd/initial.py (uniform in (0, 1)).
+__host__ __device__ inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x21F0AAADu;
+  x ^= x >> 15;
+  x *= 0x735A2D97u;
+  x ^= x >> 15;
+  return x;
+}
+
+__host__ __device__ inline double uniform01(uint32_t seed, uint32_t stream, uint32_t idx) {
+  const uint32_t key = hash32(seed ^ hash32(stream));
+  return ((double)hash32(key ^ hash32(idx)) + 0.5) / 4294967296.0;
+}
+
 // A solve has converged after this many consecutive steps with a relative energy
 // change below the tolerance (one step can dip below it by rounding noise in FP32).
 constexpr int kConvergeHits = 8;

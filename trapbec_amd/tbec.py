@@ -71,6 +71,19 @@ POTENTIALS = {"zero": 0, "harmonic": 1, "harmonic_ring": 2, "harmonic_ring2": 3,
               "mobius": 9, "table": 10}
 
 
+class TbecReferenceIc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("winding_number", C.c_int32),
+                ("vor_ring_count", C.c_int32), ("seed", C.c_uint32), ("ring_rad", C.c_double)]
+
+
+# the reference's init_cond names (src/initial.f90:58-156) and insertions (src/run.f90:26,37-40)
+IC_KINDS = {"constant": 0, "soliton": 1, "random_phase": 2, "crow": 3, "single_vortex": 4,
+            "sausage": 5, "random_points": 6, "leap_frog": 7, "vortex_rings": 8,
+            "line_of_lines": 9, "linked_rings": 10, "lattice": 11, "lattice2": 12,
+            "bundle": 13}
+INSERTS = {"pcurrent": 0, "ring": 1, "dipole": 2, "phase_imprint": 3, "noise": 4}
+
+
 class TbecError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -89,7 +102,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
            "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries",
            "tbec_set_physics", "tbec_set_potential_table", "tbec_potential", "tbec_set_time",
-           "tbec_imprint_vortex"]
+           "tbec_imprint_vortex", "tbec_reference_state", "tbec_reference_insert"]
 
 _lib = None
 
@@ -124,6 +137,8 @@ def load(path: str = LIB_PATH):
     L.tbec_potential.argtypes = [H, C.c_double, vp]
     L.tbec_set_time.argtypes = [H, C.c_double]
     L.tbec_imprint_vortex.argtypes = [H, C.c_double, C.c_double, C.c_double]
+    L.tbec_reference_state.argtypes = [H, C.POINTER(TbecReferenceIc)]
+    L.tbec_reference_insert.argtypes = [H, C.c_int, C.POINTER(TbecReferenceIc)]
     L.tbec_sync.argtypes = [H]
     L.tbec_stream.argtypes = [H]
     L.tbec_stream.restype = C.c_void_p
@@ -320,6 +335,20 @@ class Condensate:
     def imprint_vortex(self, x0: float, y0: float, direction: float):
         """force_random_points' vortex imprint at (x0, y0) with sense +1 or -1."""
         _check(load().tbec_imprint_vortex(self._h, x0, y0, direction))
+
+    def reference_state(self, kind: str, winding_number: int = 1, ring_rad: float = 4.0,
+                        vor_ring_count: int = 1, seed: int = 0):
+        """Replace psi by one of the reference's init_cond states, generated on the device."""
+        ic = TbecReferenceIc(IC_KINDS[kind], winding_number, vor_ring_count,
+                             seed & 0xFFFFFFFF, ring_rad)
+        _check(load().tbec_reference_state(self._h, C.byref(ic)))
+
+    def reference_insert(self, what: str, winding_number: int = 1, ring_rad: float = 4.0,
+                         seed: int = 0):
+        """Apply one of the reference's insertions (pcurrent, ring, dipole, phase_imprint,
+        noise) to psi on the device."""
+        ic = TbecReferenceIc(0, winding_number, 1, seed & 0xFFFFFFFF, ring_rad)
+        _check(load().tbec_reference_insert(self._h, INSERTS[what], C.byref(ic)))
 
     def set_w_storage(self, stored: bool):
         """True: RK3 keeps w in its own array (the reference's scheme, 10 transfers per
