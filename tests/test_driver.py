@@ -247,3 +247,28 @@ def test_driver_forcing_matches_oracle(tmp_path):
     s.rk3(5)
     ref = s.interior
     assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ic_crow_f64", "ins_dipole_f64", "ins_pcurrent_barrier_f64"])
+def test_driver_reference_states_match_reference(tmp_path, name):
+    """init_cond / add_pcurrent / add_dipole through the namelist, one step with dt = 0:
+    var0001.dat equals the reference's snapshot (FP64, 1e-12)."""
+    _need_driver()
+    f = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
+    shape = f["psi"].shape
+    box = tuple(float(b) for b in f["box"])
+    h = tuple(float(x) for x in f["harm_w"])
+    nml = tmp_path / "run.nml"
+    _nml(nml, Nx=shape[0], Ny=shape[1], Nz=shape[2], Lx=box[0], Ly=box[1], Lz=box[2],
+         harm_wx=h[0], harm_wy=h[1], harm_wz=h[2], init_cond=str(f["init_cond"]),
+         potential=str(f["potential"]), add_dipole=bool(f["add_dipole"]),
+         add_pcurrent=bool(f["add_pcurrent"]), dt_re=0.0, nsteps=1, shots=1,
+         out_dir=str(tmp_path / "data"))
+    print(f"\n[case] driver {name}", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, psi = _read_var(tmp_path / "data" / "proc000" / "var0001.dat", shape)
+    ref = f["psi"]
+    assert np.abs(psi - ref).max() < 1e-12 * np.abs(ref).max()

@@ -18,6 +18,8 @@ module tbec_iface
     TBEC_POT_HARMONIC_RING = 2, TBEC_POT_HARMONIC_RING2 = 3, TBEC_POT_BARRIER = 4, &
     TBEC_POT_BOX_TRAP = 5, TBEC_POT_CHANNEL = 6, TBEC_POT_QUARTIC = 7, TBEC_POT_SPHERE = 8, &
     TBEC_POT_MOBIUS = 9, TBEC_POT_TABLE = 10
+  integer(c_int), parameter :: TBEC_INSERT_PCURRENT = 0, TBEC_INSERT_RING = 1, &
+    TBEC_INSERT_DIPOLE = 2, TBEC_INSERT_PHASE_IMPRINT = 3, TBEC_INSERT_NOISE = 4
 
   type, bind(C) :: tbec_params
     integer(c_int32_t) :: nx, ny, nz
@@ -38,6 +40,12 @@ module tbec_iface
   type, bind(C) :: tbec_physics
     integer(c_int32_t) :: potential, lbeam, two_dim, pad_
     real(c_double) :: wbeam, potential_off, t_spindown, advective_vx
+  end type
+
+  type, bind(C) :: tbec_reference_ic
+    integer(c_int32_t) :: kind, winding_number, vor_ring_count
+    integer(c_int32_t) :: seed
+    real(c_double) :: ring_rad
   end type
 
   type, bind(C) :: tbec_solve_result
@@ -85,6 +93,18 @@ module tbec_iface
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h
       real(c_double), value :: x0, y0, dir
+    end function
+    integer(c_int) function tbec_reference_state(h, ic) bind(C, name="tbec_reference_state")
+      import :: c_ptr, c_int, tbec_reference_ic
+      type(c_ptr), value :: h
+      type(tbec_reference_ic), intent(in) :: ic
+    end function
+    integer(c_int) function tbec_reference_insert(h, what, ic) &
+        bind(C, name="tbec_reference_insert")
+      import :: c_ptr, c_int, tbec_reference_ic
+      type(c_ptr), value :: h
+      integer(c_int), value :: what
+      type(tbec_reference_ic), intent(in) :: ic
     end function
     integer(c_int) function tbec_velocity(h, vel) bind(C, name="tbec_velocity")
       import :: c_int, c_ptr

@@ -45,6 +45,10 @@ program tbec_run
   real(dp) :: wbeam = 4.0_dp, advective_vx = 0.0_dp
   real(dp) :: potential_off = huge(1.0_dp), t_spindown = huge(1.0_dp)
   integer :: force_freq = 10
+  ! the reference's own initial states and insertions (src/initial.f90:58-156,669-870)
+  integer :: winding_number = 1, vor_ring_count = 1
+  real(dp) :: ring_rad = 4.0_dp
+  logical :: add_pcurrent = .false., add_ring = .false., add_dipole = .false.
   character(len=256) :: trap_file = ''
   integer :: imag_nsteps = 0, imag_check = 50
   real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
@@ -55,11 +59,14 @@ program tbec_run
     ring_radius, seed, noise_amp, phase_noise_amp, imaginary_time, ic_only, &
     full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir, &
     vel_print, qpress_print, boundaries, periodicz, potential, lbeam, wbeam, &
-    potential_off, t_spindown, advective_vx, forcing, force_freq, trap_file, two_dim
+    potential_off, t_spindown, advective_vx, forcing, force_freq, trap_file, two_dim, &
+    winding_number, vor_ring_count, ring_rad, add_pcurrent, add_ring, add_dipole
 
   character(len=512) :: nml_file
   integer :: u, ios, itime, nstart, l
   integer(int64) :: nforce = 0
+  integer :: ref_kind = -1
+  type(tbec_reference_ic) :: ric
   real(dp), allocatable :: xx(:), yy(:), zz(:)
   complex(dp), allocatable, target :: psi(:, :, :)
   complex(real32), allocatable, target :: psi4(:, :, :)
@@ -120,11 +127,16 @@ program tbec_run
   case ('restart')
     call read_restart()
   case default
-    call emergency_stop('wrong initial conditions set')
+    ref_kind = ref_ic_kind()   ! generated on the device after the handle exists
   end select
+  ric%kind = max(ref_kind, 0)
+  ric%winding_number = winding_number
+  ric%vor_ring_count = vor_ring_count
+  ric%seed = int(iand(seed, 2147483647_int64), c_int32_t)
+  ric%ring_rad = ring_rad
 
   call execute_command_line('mkdir -p '//trim(out_dir)//'/proc000')
-  if (ic_only) then
+  if (ic_only .and. ref_kind < 0) then
     itime = 0
     call write_var(0)
     write(*, '(a)') ' initial state written; ic_only set, stopping'
@@ -140,12 +152,28 @@ program tbec_run
   call check(tbec_create(p, h))
   call set_bc(h)
   call set_phys(h)
-  call upload()
+  if (ref_kind >= 0) then
+    call check(tbec_reference_state(h, ric))
+    call download()
+    if (ic_only) then
+      itime = 0
+      call write_var(0)
+      write(*, '(a)') ' initial state written; ic_only set, stopping'
+      call check(tbec_destroy(h))
+      stop
+    end if
+  else
+    call upload()
+  end if
 
   if (imaginary_time .and. nstart == 1) then
     ! the imaginary-time stage needs dt = (0, -dtau): run it on a second handle
     call imaginary_stage()
   end if
+  ! src/run.f90:37-40
+  if (add_pcurrent) call check(tbec_reference_insert(h, TBEC_INSERT_PCURRENT, ric))
+  if (add_ring) call check(tbec_reference_insert(h, TBEC_INSERT_RING, ric))
+  if (add_dipole) call check(tbec_reference_insert(h, TBEC_INSERT_DIPOLE, ric))
 
   call system_clock(c_start, c_rate)
   itime = nstart - 1
@@ -308,6 +336,30 @@ contains
     close(ut)
     if (ios /= 0) call emergency_stop('short trap_file '//trim(trap_file))
   end subroutine
+
+  ! the reference's init_cond names (src/initial.f90:58-156); 'tf_vortex' and 'restart'
+  ! are handled on the host
+  integer function ref_ic_kind()
+    select case (trim(init_cond))
+    case ('constant'); ref_ic_kind = 0
+    case ('soliton'); ref_ic_kind = 1
+    case ('random_phase'); ref_ic_kind = 2
+    case ('crow'); ref_ic_kind = 3
+    case ('single_vortex'); ref_ic_kind = 4
+    case ('sausage'); ref_ic_kind = 5
+    case ('random_points'); ref_ic_kind = 6
+    case ('leap_frog'); ref_ic_kind = 7
+    case ('vortex_rings'); ref_ic_kind = 8
+    case ('line_of_lines'); ref_ic_kind = 9
+    case ('linked_rings'); ref_ic_kind = 10
+    case ('lattice'); ref_ic_kind = 11
+    case ('lattice2'); ref_ic_kind = 12
+    case ('bundle'); ref_ic_kind = 13
+    case default
+      ref_ic_kind = -1
+      call emergency_stop('wrong initial conditions set')
+    end select
+  end function
 
   ! the reference's `potential` names (src/finite_diff.f90:30-117) plus 'table'
   integer(c_int32_t) function pot_kind()
