@@ -136,6 +136,17 @@ program tbec_run
   ric%ring_rad = ring_rad
 
   call execute_command_line('mkdir -p '//trim(out_dir)//'/proc000')
+  ! dims.log as dims_print writes it (src/output.f90:48-58): one domain
+  open(newunit=u, file=trim(out_dir)//'/dims.log', status='replace')
+  write(u, *) Nx
+  write(u, *) Ny
+  write(u, *) Nz
+  write(u, *) 1
+  write(u, *) 1
+  write(u, *) Nx
+  write(u, *) Ny
+  write(u, *) Nz
+  close(u)
   if (ic_only .and. ref_kind < 0) then
     itime = 0
     call write_var(0)
