@@ -54,6 +54,10 @@ def parse():
                     default="centred")
     ap.add_argument("--variants", action="store_true",
                     help="also time the other FP32 summation modes")
+    ap.add_argument("--extensions", action="store_true",
+                    help="time the kernels beyond the headline path at n^3 FP32 (stage "
+                         "variants per physics, potential tables, reference states, "
+                         "imprint, output diagnostics) and print one JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: 3 FP32 stages at n^3 plus a plain "
                          "device copy of one psi-sized buffer (the counter calibration)")
@@ -194,6 +198,93 @@ def pmc_probe(args):
                       "stage_bytes_moved_avg": n ** 3 * 64 / 3}), flush=True)
 
 
+def extensions_probe(args):
+    """ms per step of the stage variants each physics choice selects, and the one-off
+    kernels (potential tables, the mobius Trap at (n/2)^3, reference states, the vortex
+    imprint, velocity / quantum pressure / vortex cores with their host copies) at n^3
+    FP32, centred sums, w rebuilt, tuned tiles.  Step timings use HIP events on the
+    handle's stream; calls that synchronise are timed on the host clock."""
+    import torch
+    from trapbec_amd import initial, tbec, autotune
+    n = args.n
+    pr, spec = initial.ic_spec("m", n)
+    out = {"extensions": f"{n}^3 FP32"}
+
+    def handle():
+        c = tbec.Condensate.from_problem(pr, prec=32, sum_mode=2)
+        tile, seg, remap = autotune.lookup(pr.shape, 32, 2, False, False)
+        c.set_tuning(tile, seg)
+        c.set_xcd_remap(remap)
+        c.init_state(spec)
+        return c
+
+    def step_ms(c, steps=40, warm=4):
+        c.step(warm)
+        c.sync()
+        stream = torch.cuda.ExternalStream(c.stream_ptr())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        c.step(steps)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    def table(c):
+        t = c.potential(0.0)
+        c.set_physics("table")
+        c.set_potential_table(t)
+
+    variants = [("harmonic_analytic", lambda c: None),
+                ("harmonic_as_table", table),
+                ("channel_static_table", lambda c: c.set_physics("channel")),
+                ("harmonic_advective", lambda c: c.set_physics("harmonic", advective_vx=0.3)),
+                ("harmonic_ring_closing", lambda c: (c.set_physics("harmonic_ring"),
+                                                     setattr(c, "t", 49.99))),
+                ("box_trap_refresh_every_step", lambda c: c.set_physics("box_trap"))]
+    steps = {}
+    for name, setup in variants:
+        c = handle()
+        setup(c)
+        steps[name] = round(step_ms(c), 4)
+        print(f"[ext] {name}: {steps[name]} ms/step", file=sys.stderr, flush=True)
+        c.close()
+    out["ms_per_step"] = steps
+    once = {}
+    c = handle()
+    for kind in ("constant", "lattice", "vortex_rings", "random_phase"):
+        once[f"reference_state_{kind}"] = round(wall(lambda: c.reference_state(
+            kind, vor_ring_count=3, seed=1)), 3)
+    c.init_state(spec)
+    stream = torch.cuda.ExternalStream(c.stream_ptr())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    c.imprint_vortex(1.0, -2.0, 1.0)
+    e1.record(stream)
+    e1.synchronize()
+    once["imprint_vortex"] = round(e0.elapsed_time(e1), 3)
+    once["potential_table_to_host"] = round(wall(lambda: c.potential(0.0)), 3)
+    once["velocity_with_host_copy"] = round(wall(c.velocity), 3)
+    once["qpressure_with_host_copy"] = round(wall(c.qpressure), 3)
+    once["vortex_cores_z"] = round(wall(lambda: c.vortex_cores(1e-5 * 10.0 / 500.0)), 3)
+    c.close()
+    h = n // 2
+    pr2 = initial.Problem((h, h, h), (64.0, 64.0, 16.0), (1.0, 1.0, 1.0), 500.0, 0.0, 5e-4)
+    c = tbec.Condensate.from_problem(pr2, prec=32, sum_mode=2)
+    once[f"mobius_trap_{h}^3"] = round(wall(lambda: c.set_physics("mobius")), 3)
+    c.close()
+    out["ms_once"] = once
+    for k, v in once.items():
+        print(f"[ext] {k}: {v} ms", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,6 +299,9 @@ def main():
     n = args.n
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.extensions:
+        extensions_probe(args)
         return
     pr, spec = initial.ic_spec("m", n)
     res = {}

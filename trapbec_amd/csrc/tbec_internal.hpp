@@ -106,6 +106,7 @@ struct tbec_handle {
   double t_now = 0.0;                   // the handle's time t (real-time steps advance it)
   void *vtab = nullptr;                 // tabulated potential: padded layout, one real per point
   void *trap = nullptr;                 // Trap table (mobius or the caller's): interior layout
+  void *coords = nullptr;               // cell-centre coordinates x, y, z (working precision)
   int trap_owner = 0;                   // kind whose Trap the table holds (MOBIUS / TABLE), 0 none
   bool vtab_valid = false;
   double vkey[5] = {0, 0, 0, 0, 0};     // time key of the table held in vtab

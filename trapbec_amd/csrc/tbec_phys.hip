@@ -44,6 +44,7 @@ struct PotArgs {
   double s, b, las_x, las_y;     // T values formed on the host
   double wbeam;
   const void *trap;              // interior layout, T
+  const void *coords;            // xx(1:nx), yy(1:ny), zz(1:nz) in T (src/initial.f90:34-42)
 };
 
 // Cell-centred coordinate of 0-based index i on an axis of n points and length L
@@ -56,7 +57,8 @@ template <typename T> __device__ __forceinline__ T coord(int i, int n, T L) {
 template <typename T> __device__ T pot_point(const PotArgs &a, int i, int j, int k) {
 #pragma clang fp contract(off)
   const T Lx = (T)a.Lx, Ly = (T)a.Ly, Lz = (T)a.Lz;
-  const T x = coord<T>(i, a.nx, Lx), y = coord<T>(j, a.ny, Ly), z = coord<T>(k, a.nz, Lz);
+  const T *cc = (const T *)a.coords;
+  const T x = cc[i], y = cc[a.nx + j], z = cc[a.nx + a.ny + k];
   const T s = (T)a.s;
   T pot = (T)0;
   switch (a.kind) {
@@ -377,6 +379,41 @@ PotArgs pot_args(const tbec_handle *h, double t, bool pot_on) {
   return a;
 }
 
+// The cell-centre coordinates in the working precision, formed on the host with the
+// same IEEE operations the reference uses and kept on the device (one table per handle).
+template <typename T> std::vector<T> coord_table(const tbec_handle *h) {
+  std::vector<T> c;
+  const int n[3] = {h->p.nx, h->p.ny, h->p.nz};
+  const T L[3] = {(T)h->p.Lx, (T)h->p.Ly, (T)h->p.Lz};
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < n[a]; ++i)
+      c.push_back(((T)(2 * i + 1) / ((T)2 * (T)n[a])) * L[a] - L[a] / (T)2);
+  return c;
+}
+
+int ensure_coords(tbec_handle *h) {
+  if (h->coords) return 0;
+  const size_t cnt = (size_t)h->p.nx + h->p.ny + h->p.nz;
+  const size_t bytes = cnt * (h->csize / 2);
+  hipError_t e = hipMalloc(&h->coords, bytes);
+  if (e == hipSuccess) {
+    if (is_f32(h)) {
+      const std::vector<float> c = coord_table<float>(h);
+      e = hipMemcpy(h->coords, c.data(), bytes, hipMemcpyHostToDevice);
+    } else {
+      const std::vector<double> c = coord_table<double>(h);
+      e = hipMemcpy(h->coords, c.data(), bytes, hipMemcpyHostToDevice);
+    }
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (h->coords) hipFree(h->coords);
+    h->coords = nullptr;
+    return set_error(TBEC_ERR_OOM, std::string("coordinate table: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
 // Fill `table` (padded layout, real T) with pot at time t.
 int eval_table(tbec_handle *h, double t, bool pot_on, void *table) {
   const int k = h->phys.potential;
@@ -386,7 +423,10 @@ int eval_table(tbec_handle *h, double t, bool pot_on, void *table) {
                                          : "the mobius Trap was replaced; select it again");
   if (h->p.nx > 65535 || h->p.ny > 65535)
     return set_error(TBEC_ERR_ARG, "tabulated potentials need nx, ny <= 65535");
-  const PotArgs a = pot_args(h, t, pot_on);
+  int rc = ensure_coords(h);
+  if (rc) return rc;
+  PotArgs a = pot_args(h, t, pot_on);
+  a.coords = h->coords;
   const dim3 grid((unsigned)((h->p.nz + 255) / 256), (unsigned)h->p.ny, (unsigned)h->p.nx);
   if (is_f32(h))
     hipLaunchKernelGGL(k_potential<float>, grid, dim3(256), 0, h->stream, a, (float *)table);
@@ -540,7 +580,8 @@ int prepare_physics(tbec_handle *h, double t) {
 void free_physics(tbec_handle *h) {
   if (h->vtab) hipFree(h->vtab);
   if (h->trap) hipFree(h->trap);
-  h->vtab = h->trap = nullptr;
+  if (h->coords) hipFree(h->coords);
+  h->vtab = h->trap = h->coords = nullptr;
 }
 
 }  // namespace tbec
