@@ -44,31 +44,36 @@ def _run(cmd):
     return cmd
 
 
-def build_lib(verbose: bool = False, jobs: int = 0) -> str:
+def build_lib(verbose: bool = False, jobs: int = 0, extra=(), out: str = None) -> str:
+    """Compile libtbec for gfx950 (to `out`, default trapbec_amd/libtbec.so); `extra` are
+    additional compiler flags."""
     cc = hipcc()
+    extra = list(extra)
+    lib = out or LIB
     jobs = jobs or min(8, os.cpu_count() or 1)
     tmpd = tempfile.mkdtemp(prefix="tbec_build_")
     try:
         cmds = []
         for s in SOURCES:
             src = os.path.join(CSRC, s)
-            if os.path.exists(src):
-                cmds.append([cc, *HIPFLAGS, "-c", src, "-o", os.path.join(tmpd, s + ".o")])
+            if not os.path.exists(src):
+                raise FileNotFoundError(f"libtbec source {src} is missing")
+            cmds.append([cc, *HIPFLAGS, *extra, "-c", src, "-o", os.path.join(tmpd, s + ".o")])
         inst = os.path.join(CSRC, "stage_inst.hip")
         for p, t in INSTANCES:
-            cmds.append([cc, *HIPFLAGS, f"-DINST_PREC={p}", f"-DINST_TILE={t}", "-c", inst,
+            cmds.append([cc, *HIPFLAGS, *extra, f"-DINST_PREC={p}", f"-DINST_TILE={t}", "-c", inst,
                          "-o", os.path.join(tmpd, f"stage_p{p}_t{t}.o")])
         with ThreadPoolExecutor(jobs) as ex:
             for cmd in ex.map(_run, cmds):
                 if verbose:
                     print(" ".join(cmd), flush=True)
         objs = [c[-1] for c in cmds]
-        tmp = LIB + ".tmp"
+        tmp = lib + ".tmp"
         _run([cc, "--offload-arch=gfx950", "-shared", *objs, "-o", tmp])
-        os.replace(tmp, LIB)
+        os.replace(tmp, lib)
     finally:
         shutil.rmtree(tmpd, ignore_errors=True)
-    return LIB
+    return lib
 
 
 if __name__ == "__main__":
