@@ -330,7 +330,7 @@ __device__ __forceinline__ void block_sum4(double v[4], double *out) {
 // ---- the stage kernel ----------------------------------------------------------
 // Process one tile (TZ x TY in z, y; one x segment) of one condensate: logical tile id
 // lb in [0, tiles_z*tiles_y*nseg).  Accumulates this thread's reduction terms in red.
-template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG, int SK = -1>
 __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, double red[4]) {
   using C = cplx<T>;
   constexpr int NTZ = TZ / VZ;
@@ -437,7 +437,10 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   if (hl) ldh(rs(pin + (long long)xs * sx), hoffb, hb[0]);
   // w of the first plane (stages that read w); later planes are prefetched one ahead
   C wq[2][VZ];
-  if (stage_reads_w(a.stage))
+  // SK >= 0: the stage kind is fixed at compile time (real time, w rebuilt), which removes
+  // the run-time stage branches from the plane loop
+  constexpr bool FIX = SK >= 0;
+  if (stage_reads_w(FIX ? SK : a.stage))
     BufIO<T, VZ>::template ld<kPointwiseLoadCpol>(rs(p_w), ownb, wq[0]);
   // tabulated potential (PH_GEN): own points of plane x, prefetched one plane ahead
   constexpr bool VT = PHYS == PH_GEN;
@@ -456,9 +459,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const T gnl = T(a.gnl), om = T(a.om);
   const T cre = T(a.cre), cim = T(a.cim), A = T(a.A), B = T(a.B);
   const T wrec = T(a.wrec);
-  const bool rebuild = a.wrec != 0.0;
-  const bool store_w = a.store_w != 0;
-  const int stage = a.stage;
+  const bool rebuild = FIX ? (SK == ST_RK2 || SK == ST_RK3) : a.wrec != 0.0;
+  const bool store_w = FIX ? false : a.store_w != 0;
+  const int stage = FIX ? SK : a.stage;
   const T yc = T(a.y0 + a.hy * y);
   T zc2[VZ];
 #pragma unroll
@@ -649,7 +652,7 @@ template <typename T, int NT, bool IMAG> constexpr int stage_min_waves() {
   return (sizeof(T) == 4 && NT == 512) ? 4 : 1;
 }
 
-template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG, int SK = -1>
 __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage(const StageArgs a) {
   constexpr int NT = (TZ / VZ) * TY;
@@ -663,7 +666,7 @@ gpe_stage(const StageArgs a) {
   int lb = b;
   if (a.remap && (nb & 7) == 0) lb = (b & 7) * (nb >> 3) + (b >> 3);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG>(a, lb, red);
+  march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG, SK>(a, lb, red);
   if constexpr (IMAG) {
     if (a.partials) block_sum4<NT>(red, a.partials + 4LL * lb);
   }

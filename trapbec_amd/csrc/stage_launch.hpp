@@ -55,12 +55,43 @@ TBEC_DECL_BLAUNCHER(64, 4)
 hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool rot, int grid,
                         hipStream_t s);
 
+// Real-time stages with w rebuilt in the default summation of each precision (centred
+// FP32, reference FP64) also come as instances with the stage kind fixed at compile time:
+// the plane loop then carries no stage branches (measured at 512^3: FP32 -8.5% per step,
+// FP32 rotating -8.2%, FP64 rotating -4%).  Every other combination (stored w, imaginary
+// time, the other FP32 sums) uses the run-time stage kind.
+#ifndef TBEC_STAGE_SPECIALISE
+#define TBEC_STAGE_SPECIALISE 1
+#endif
+template <typename T, int S, int PH> constexpr bool stage_specialised() {
+  return TBEC_STAGE_SPECIALISE &&
+         ((sizeof(T) == 4 && S == SUM_CENTRED) || (sizeof(T) == 8 && S == SUM_REF));
+}
+
 template <typename T, int TZ, int TY, int VZ, int S, int PH>
 static void launch_phys(const StageArgs &a, bool imag, hipStream_t s) {
   constexpr int NT = (TZ / VZ) * TY;
   dim3 grid(a.nblocks), block(NT);
-  if (imag) hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false>), grid, block, 0, s, a);
+  if (imag) {
+    hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true>), grid, block, 0, s, a);
+    return;
+  }
+  if constexpr (stage_specialised<T, S, PH>()) {
+    const bool rebuilt = !a.store_w && (a.stage == ST_RK1 || a.wrec != 0.0);
+    if (rebuilt && a.stage == ST_RK1) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK1>), grid, block, 0, s, a);
+      return;
+    }
+    if (rebuilt && a.stage == ST_RK2) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK2>), grid, block, 0, s, a);
+      return;
+    }
+    if (rebuilt && a.stage == ST_RK3) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK3>), grid, block, 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false>), grid, block, 0, s, a);
 }
 
 template <typename T, int TZ, int TY, int VZ, int S>
