@@ -696,7 +696,7 @@ struct BatchLaunch {
   const int *flags;     // per-instance flags (stride 4); skip when flags[4*i] != 0
 };
 
-template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG>
+template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG, int SK = -1>
 __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage_batch(const BatchLaunch L) {
   constexpr int NT = (TZ / VZ) * TY;
@@ -721,7 +721,7 @@ gpe_stage_batch(const BatchLaunch L) {
     a.c0dev = IMAG ? st + 1 : nullptr;                            // ST_C0EFF
     a.skip = nullptr;
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG>(a, item - bi.item_begin, red);
+    march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG, SK>(a, item - bi.item_begin, red);
     if constexpr (IMAG) block_sum4<NT>(red, L.partials + 4LL * item);
     __syncthreads();
   }

@@ -55,13 +55,18 @@ TBEC_DECL_BLAUNCHER(64, 4)
 hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool rot, int grid,
                         hipStream_t s);
 
-// Real-time stages with w rebuilt in the default summation of each precision (centred
-// FP32, reference FP64) also come as instances with the stage kind fixed at compile time:
-// the plane loop then carries no stage branches (measured at 512^3: FP32 -8.5% per step,
-// FP32 rotating -8.2%, FP64 rotating -4%).  Every other combination (stored w, imaginary
-// time, the other FP32 sums) uses the run-time stage kind.
+// Stages with w rebuilt in the default summation of each precision (centred FP32,
+// reference FP64) also come as instances with the stage kind fixed at compile time, in
+// real and imaginary time and in the batch kernel: the plane loop then carries no stage
+// branches and the imaginary-time variants spill less (measured: real time at 512^3
+// FP32 -8.5% per step, FP64 rotating -4%; imaginary time at 256^3 rotating FP32 -12%,
+// FP64 -18%; a batch of 12 x 96^3 -21% / -16%).  Stored w and the other FP32 sums use the
+// run-time stage kind.
 #ifndef TBEC_STAGE_SPECIALISE
 #define TBEC_STAGE_SPECIALISE 1
+#endif
+#ifndef TBEC_STAGE_SPECIALISE_IMAG
+#define TBEC_STAGE_SPECIALISE_IMAG 1
 #endif
 template <typename T, int S, int PH> constexpr bool stage_specialised() {
   return TBEC_STAGE_SPECIALISE &&
@@ -73,6 +78,21 @@ static void launch_phys(const StageArgs &a, bool imag, hipStream_t s) {
   constexpr int NT = (TZ / VZ) * TY;
   dim3 grid(a.nblocks), block(NT);
   if (imag) {
+    if constexpr (stage_specialised<T, S, PH>() && TBEC_STAGE_SPECIALISE_IMAG) {
+      const bool rebuilt = !a.store_w && (a.stage == ST_RK1 || a.wrec != 0.0);
+      if (rebuilt && a.stage == ST_RK1) {
+        hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true, ST_RK1>), grid, block, 0, s, a);
+        return;
+      }
+      if (rebuilt && a.stage == ST_RK2) {
+        hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true, ST_RK2>), grid, block, 0, s, a);
+        return;
+      }
+      if (rebuilt && a.stage == ST_RK3) {
+        hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true, ST_RK3>), grid, block, 0, s, a);
+        return;
+      }
+    }
     hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, true>), grid, block, 0, s, a);
     return;
   }
@@ -116,15 +136,36 @@ static hipError_t launch_tile(const StageArgs &a, int sum, int phys, bool imag,
 }
 
 // Batched imaginary-time stages (reductions always on).
+template <typename T, int TZ, int TY, int VZ, int S, int PH>
+static void launch_batch_phys(const BatchLaunch &L, int grid, hipStream_t s) {
+  constexpr int NT = (TZ / VZ) * TY;
+  // batches always rebuild w: the three RK stage kinds have their own instances in the
+  // default summations
+  if constexpr (stage_specialised<T, S, PH>() && TBEC_STAGE_SPECIALISE_IMAG) {
+    if (L.stage == ST_RK1) {
+      hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH, true, ST_RK1>), dim3(grid),
+                         dim3(NT), 0, s, L);
+      return;
+    }
+    if (L.stage == ST_RK2) {
+      hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH, true, ST_RK2>), dim3(grid),
+                         dim3(NT), 0, s, L);
+      return;
+    }
+    if (L.stage == ST_RK3) {
+      hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH, true, ST_RK3>), dim3(grid),
+                         dim3(NT), 0, s, L);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH, true>), dim3(grid), dim3(NT), 0, s,
+                     L);
+}
+
 template <typename T, int TZ, int TY, int VZ, int S>
 static void launch_batch_sum(const BatchLaunch &L, bool rot, int grid, hipStream_t s) {
-  constexpr int NT = (TZ / VZ) * TY;
-  if (rot)
-    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH_ROT, true>), dim3(grid), dim3(NT),
-                       0, s, L);
-  else
-    hipLaunchKernelGGL((gpe_stage_batch<T, TZ, TY, VZ, S, PH_HARM, true>), dim3(grid),
-                       dim3(NT), 0, s, L);
+  if (rot) launch_batch_phys<T, TZ, TY, VZ, S, PH_ROT>(L, grid, s);
+  else launch_batch_phys<T, TZ, TY, VZ, S, PH_HARM>(L, grid, s);
 }
 
 template <typename T, int TZ, int TY, int VZ>
