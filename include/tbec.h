@@ -261,7 +261,8 @@ int tbec_set_physics(tbec_handle *h, const tbec_physics *ph);
 /* The Trap table for TBEC_POT_TABLE (nx*ny*nz reals of the working precision, z
  * fastest, host buffer). */
 int tbec_set_potential_table(tbec_handle *h, const void *trap_host);
-/* pot over the interior at time t (nx*ny*nz reals, z fastest, host buffer); synchronises. */
+/* pot over the interior at time t (nx*ny*nz reals, z fastest, host buffer); synchronises.
+ * Uses the handle's w array as scratch (w carries nothing across steps). */
 int tbec_potential(tbec_handle *h, double t, void *pot_host);
 /* The handle's time t: real-time steps start from it and advance it; imaginary-time
  * steps and the diagnostics evaluate the potential at it (0 after creation, as the

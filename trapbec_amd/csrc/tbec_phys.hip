@@ -407,7 +407,7 @@ int ensure_coords(tbec_handle *h) {
   }
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    if (h->coords) hipFree(h->coords);
+    if (h->coords) (void)hipFree(h->coords);
     h->coords = nullptr;
     return set_error(TBEC_ERR_OOM, std::string("coordinate table: ") + hipGetErrorString(e));
   }
@@ -506,6 +506,7 @@ template <> std::vector<float> mobius_strip<float>() {
 template <typename T> int build_mobius(tbec_handle *h) {
   int rc = ensure_trap(h);
   if (rc) return rc;
+  h->trap_owner = 0;   // the buffer is rewritten below: no table is valid until it ends
   const std::vector<T> strip = mobius_strip<T>();
   T *dstrip = nullptr;
   hipError_t e = hipMalloc(&dstrip, strip.size() * sizeof(T));
@@ -529,8 +530,8 @@ template <typename T> int build_mobius(tbec_handle *h) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(dstrip);
-  if (part) hipFree(part);
+  (void)hipFree(dstrip);
+  if (part) (void)hipFree(part);
   if (e != hipSuccess)
     return set_error(TBEC_ERR_HIP, std::string("mobius Trap: ") + hipGetErrorString(e));
   h->trap_owner = TBEC_POT_MOBIUS;
@@ -578,9 +579,9 @@ int prepare_physics(tbec_handle *h, double t) {
 }
 
 void free_physics(tbec_handle *h) {
-  if (h->vtab) hipFree(h->vtab);
-  if (h->trap) hipFree(h->trap);
-  if (h->coords) hipFree(h->coords);
+  if (h->vtab) (void)hipFree(h->vtab);
+  if (h->trap) (void)hipFree(h->trap);
+  if (h->coords) (void)hipFree(h->coords);
   h->vtab = h->trap = h->coords = nullptr;
 }
 
@@ -599,12 +600,16 @@ int tbec_set_physics(tbec_handle *h, const tbec_physics *ph) {
     return set_error(TBEC_ERR_ARG, "switch-off times must not be NaN; vx must be finite");
   int rc = tbec_materialize_scale(h);
   if (rc) return rc;
+  const tbec_physics before = h->phys;
   h->phys = *ph;
   h->vtab_valid = false;
   h->ph_sig = -1;
   tbec_drop_graphs(h);
-  if (ph->potential == TBEC_POT_MOBIUS && h->trap_owner != TBEC_POT_MOBIUS)
-    return is_f32(h) ? build_mobius<float>(h) : build_mobius<double>(h);
+  if (ph->potential == TBEC_POT_MOBIUS && h->trap_owner != TBEC_POT_MOBIUS) {
+    rc = is_f32(h) ? build_mobius<float>(h) : build_mobius<double>(h);
+    if (rc) h->phys = before;   // the handle keeps its previous physics
+    return rc;
+  }
   return 0;
 }
 
@@ -653,7 +658,7 @@ int tbec_potential(tbec_handle *h, double t, void *pot_host) {
     e = hipMemcpyAsync(pot_host, dout, (size_t)n * (h->csize / 2), hipMemcpyDeviceToHost,
                        h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(dout);
+  (void)hipFree(dout);
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, std::string("potential: ") +
                                                           hipGetErrorString(e));
   return 0;
