@@ -20,6 +20,10 @@ writes w and psi2; stage 3 reads psi2 and w, writes psi.  `roofline.achieved` an
 `frac` use that basis for every variant.  With w rebuilt (stage 2 reads psi1 and psi0,
 stage 3 reads psi2 and psi1, no w) the kernels need only 8 (64 B / 128 B);
 `roofline.frac_of_bytes_moved` states the fraction on that basis.
+
+`stream_ceiling` is the same step's read/write mix moved by plain streaming kernels
+(torch's copy and add over n^3 complex arrays, timed here): the rate HBM actually delivers
+for that mix on this GPU, and `stream_ceiling.frac` the stepper's value against it.
 """
 from __future__ import annotations
 
@@ -118,6 +122,43 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False, table=Fa
     return {"ms_per_step": ms, "gpt_steps_per_s": gpts, "wall_s": wall,
             "achieved_Bps": achieved, "moved_Bps": moved, "tile": tile, "seg": seg,
             "remap": remap, "ms_per_launch": ms / 3.0, "w_stored": w_stored}
+
+
+def stream_ceiling(n, prec, reps=20):
+    """The read/write mix of one RK3 step with w rebuilt, streamed by torch's own
+    elementwise kernels over n^3 complex arrays: stage 1 reads one array and writes one (a
+    copy), stages 2 and 3 read two and write one (an add).  copy + 2 x add is the time a
+    step would take if its stages moved only their algorithmic bytes at the rate plain
+    streaming kernels reach on this GPU.  Timed with events on torch's stream."""
+    import torch
+    dt = torch.complex64 if prec == 32 else torch.complex128
+    x = torch.ones(n ** 3, dtype=dt, device="cuda:0")
+    y = torch.ones_like(x)
+    z = torch.empty_like(x)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    copy_ms = timed(lambda: z.copy_(x))
+    add_ms = timed(lambda: torch.add(x, y, out=z))
+    del x, y, z
+    torch.cuda.empty_cache()
+    step_ms = copy_ms + 2.0 * add_ms
+    esize = 8 if prec == 32 else 16
+    return {"copy_us": round(copy_ms * 1e3, 1), "add_us": round(add_ms * 1e3, 1),
+            "copy_GBps": round(2 * esize * n ** 3 / (copy_ms * 1e-3) / 1e9, 1),
+            "add_GBps": round(3 * esize * n ** 3 / (add_ms * 1e-3) / 1e9, 1),
+            "step_ms": round(step_ms, 5),
+            "value": round(n ** 3 / (step_ms * 1e-3) / 1e9, 4)}
 
 
 def cpu_baseline(n):
@@ -337,6 +378,9 @@ def main():
                 print(f"[bench] fp32 {name}: {v['ms_per_step']:.3f} ms/step", file=sys.stderr,
                       flush=True)
         torch.cuda.empty_cache()
+        res.setdefault("stream", {})[prec] = stream_ceiling(n, prec)
+        print(f"[bench] fp{prec} stream ceiling: {res['stream'][prec]['step_ms']:.3f} "
+              "ms/step", file=sys.stderr, flush=True)
     head = 32 if 32 in res else 64
     r = res[head]
     out = {
@@ -376,8 +420,14 @@ def main():
                 "frac_of_bytes_moved": round(v["moved_Bps"] / HBM_PEAK, 4),
                 "achieved_GBps": round(v["achieved_Bps"] / 1e9, 1),
                 "tile": v["tile"], "seg": v["seg"], "steps": steps}
+    sc = res["stream"][head]
+    out["stream_ceiling"] = dict(sc, frac=round(out["value"] / sc["value"], 4),
+                                 basis="torch copy (stage 1) + 2 x torch add (stages 2, 3) "
+                                       f"over {n}^3 complex arrays")
     if 64 in res and head == 32:
         out["fp64"] = brief(res[64], args.fp64_steps)
+        out["fp64"]["stream_ceiling"] = dict(res["stream"][64], frac=round(
+            out["fp64"]["value"] / res["stream"][64]["value"], 4))
     for prec, v in res.get("stored", {}).items():
         out[f"fp{prec}_stored_w"] = brief(v, args.steps if prec == 32 else args.fp64_steps)
     if "rot" in res:
