@@ -365,11 +365,21 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   C *pout = reinterpret_cast<C *>(a.psi_out) + rb;
   C *pw = reinterpret_cast<C *>(a.w) + rb;
 
+  // Imaginary time: stage 1 reads psi as scale*psi, stage 2 forms the same product of
+  // psi_{s-1}; the host sets scale / prev_scale only for those stages.  Instances with the
+  // stage kind fixed carry only the one they use (a multiply by 1 is exact, so the other
+  // instances give the same results).
+  constexpr bool USE_SC = IMAG && (SK < 0 || SK == ST_RK1);
+  constexpr bool USE_PSC = IMAG && (SK < 0 || SK == ST_RK2);
   T sc = T(1), psc = T(1);
   T c0 = T(a.c0);
   if constexpr (IMAG) {
-    if (a.scale) sc = T(*a.scale);
-    if (a.prev_scale) psc = T(*a.prev_scale);
+    if constexpr (USE_SC) {
+      if (a.scale) sc = T(*a.scale);
+    }
+    if constexpr (USE_PSC) {
+      if (a.prev_scale) psc = T(*a.prev_scale);
+    }
     if (a.c0dev) c0 = T(*a.c0dev);
   }
 
@@ -419,7 +429,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   C *p_w = pw + (long long)xs * sx;
   const uint32_t ownb = own * (uint32_t)sizeof(C), hoffb = hoff * (uint32_t)sizeof(C);
   auto scale_in = [&](C *o) {
-    if constexpr (IMAG) {
+    if constexpr (USE_SC) {
 #pragma unroll
       for (int v = 0; v < VZ; ++v) { o[v].re *= sc; o[v].im *= sc; }
     }
@@ -470,6 +480,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     zc2[v] = wlz * (zc * zc);
   }
   const T poty = wly * (yc * yc);
+  const T omyk = om * yc * k1x;     // FP32 frame term: om * y / (60 hx)
   bool zin[VZ];
   bool full = true;
 #pragma unroll
@@ -509,6 +520,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
 
     const T xc = T(a.x0 + a.hx * x);
     const T potxy = wlx * (xc * xc) + poty;
+    const T omxk = om * xc * k1y;   // FP32 frame term: om * x / (60 hy)
     // z row segment: columns tz*VZ .. tz*VZ+VZ+7  (z-4 .. z+VZ+3)
     C zr[VZ + 8];
     {
@@ -540,14 +552,25 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         ri = (dxi * k2x + dyi * k2y) + dzi * k2z;
       }
       if (PHYS == PH_ROT || drift == 1) {
-        const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
-        const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
-        const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
-        const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
-        const T pr = xc * d1yr - yc * d1xr;
-        const T pim = xc * d1yi - yc * d1xi;
-        rr -= om * pim;   // (0, om) * (pr, pim)
-        ri += om * pr;
+        if constexpr (sizeof(T) == 4) {
+          // FP32: om*x/(60 hy) and om*y/(60 hx) folded into one factor each, so the
+          // frame term is two FMAs per component (fewer live registers)
+          const T s1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re);
+          const T s1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im);
+          const T s1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re);
+          const T s1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im);
+          rr = (rr - omxk * s1yi) + omyk * s1xi;   // (0, om) * (x D1y - y D1x)
+          ri = (ri + omxk * s1yr) - omyk * s1xr;
+        } else {
+          const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
+          const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
+          const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
+          const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
+          const T pr = xc * d1yr - yc * d1xr;
+          const T pim = xc * d1yi - yc * d1xi;
+          rr -= om * pim;   // (0, om) * (pr, pim)
+          ri += om * pr;
+        }
       } else if (VT && drift == 2) {
         const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
         const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
@@ -571,7 +594,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
       } else {
         C wp = wv[v];
         if (rebuild) {
-          if constexpr (IMAG) { wp.re *= psc; wp.im *= psc; }
+          if constexpr (USE_PSC) { wp.re *= psc; wp.im *= psc; }
           wp = {(f.re - wp.re) * wrec, (f.im - wp.im) * wrec};
         }
         wo[v] = {A * wp.re + ur, A * wp.im + ui};
