@@ -388,7 +388,7 @@ bool handle_rot(const tbec_handle *h) { return rot_enabled(h->p); }
 
 int tbec_reserve_partials(tbec_handle *h, int nb) {
   if (nb > h->npart) {
-    if (h->partials) hipFree(h->partials);
+    if (h->partials) (void)hipFree(h->partials);
     h->partials = nullptr;
     HIPCHK(hipMalloc(&h->partials, sizeof(double) * 4 * (size_t)nb));
     h->npart = nb;
@@ -541,18 +541,18 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
 int tbec_destroy(tbec_handle *h) {
   if (!h) return 0;
   tbec_drop_graphs(h);
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_physics(h);
-  if (h->psi[0]) hipFree(h->psi[0]);
-  if (h->psi[1]) hipFree(h->psi[1]);
-  if (h->w) hipFree(h->w);
-  if (h->stage) hipFree(h->stage);
-  if (h->partials) hipFree(h->partials);
-  if (h->dstate) hipFree(h->dstate);
-  if (h->dflag) hipFree(h->dflag);
-  if (h->hstate) hipHostFree(h->hstate);
-  if (h->hflag) hipHostFree(h->hflag);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->psi[0]) (void)hipFree(h->psi[0]);
+  if (h->psi[1]) (void)hipFree(h->psi[1]);
+  if (h->w) (void)hipFree(h->w);
+  if (h->stage) (void)hipFree(h->stage);
+  if (h->partials) (void)hipFree(h->partials);
+  if (h->dstate) (void)hipFree(h->dstate);
+  if (h->dflag) (void)hipFree(h->dflag);
+  if (h->hstate) (void)hipHostFree(h->hstate);
+  if (h->hflag) (void)hipHostFree(h->hflag);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
 }

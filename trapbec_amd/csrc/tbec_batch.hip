@@ -157,10 +157,10 @@ int build_table(tbec_batch *b) {
   std::vector<int> item_inst(b->total_items);
   for (int i = 0; i < b->n; ++i)
     for (int k = b->item_begin[i]; k < b->item_begin[i + 1]; ++k) item_inst[k] = i;
-  if (b->d_inst) hipFree(b->d_inst);
-  if (b->d_item_inst) hipFree(b->d_item_inst);
-  if (b->d_item_begin) hipFree(b->d_item_begin);
-  if (b->d_part) hipFree(b->d_part);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_item_inst) (void)hipFree(b->d_item_inst);
+  if (b->d_item_begin) (void)hipFree(b->d_item_begin);
+  if (b->d_part) (void)hipFree(b->d_part);
   b->d_inst = nullptr;
   b->d_item_inst = b->d_item_begin = nullptr;
   b->d_part = nullptr;
@@ -178,7 +178,7 @@ int build_table(tbec_batch *b) {
 
 void drop_graphs(tbec_batch *b) {
   for (auto &g : b->graph)
-    if (g) { hipGraphExecDestroy(g); g = nullptr; }
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   b->graph_steps = 0;
 }
 
@@ -232,12 +232,12 @@ int replay_batch(tbec_batch *b, long long n) {
     hipError_t e = hipStreamEndCapture(b->stream, &gr);
     b->parity = p0;
     if (rc) {
-      if (gr) hipGraphDestroy(gr);
+      if (gr) (void)hipGraphDestroy(gr);
       return rc;
     }
     if (e != hipSuccess) return set_error(TBEC_ERR_HIP, "batch graph capture failed");
     e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
-    hipGraphDestroy(gr);
+    (void)hipGraphDestroy(gr);
     if (e != hipSuccess) return set_error(TBEC_ERR_HIP, "batch graph instantiate failed");
     b->graph_steps = n;
   }
@@ -341,18 +341,18 @@ int tbec_batch_create(int32_t n, const tbec_params *params, tbec_batch **out) {
 int tbec_batch_destroy(tbec_batch *b) {
   if (!b) return 0;
   drop_graphs(b);
-  if (b->stream) hipStreamSynchronize(b->stream);
-  for (void *p : b->bufs) hipFree(p);
-  if (b->stage) hipFree(b->stage);
-  if (b->d_inst) hipFree(b->d_inst);
-  if (b->d_item_inst) hipFree(b->d_item_inst);
-  if (b->d_item_begin) hipFree(b->d_item_begin);
-  if (b->d_part) hipFree(b->d_part);
-  if (b->d_state) hipFree(b->d_state);
-  if (b->d_flags) hipFree(b->d_flags);
-  if (b->h_state) hipHostFree(b->h_state);
-  if (b->h_flags) hipHostFree(b->h_flags);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void *p : b->bufs) (void)hipFree(p);
+  if (b->stage) (void)hipFree(b->stage);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_item_inst) (void)hipFree(b->d_item_inst);
+  if (b->d_item_begin) (void)hipFree(b->d_item_begin);
+  if (b->d_part) (void)hipFree(b->d_part);
+  if (b->d_state) (void)hipFree(b->d_state);
+  if (b->d_flags) (void)hipFree(b->d_flags);
+  if (b->h_state) (void)hipHostFree(b->h_state);
+  if (b->h_flags) (void)hipHostFree(b->h_flags);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
 }

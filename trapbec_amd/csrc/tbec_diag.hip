@@ -136,7 +136,7 @@ int run_field(tbec_handle *h, bool velocity, void *out_host) {
     e = hipMemcpyAsync(out_host, dout, sizeof(T) * 3 * (size_t)n, hipMemcpyDeviceToHost,
                        h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(dout);
+  (void)hipFree(dout);
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, std::string("diagnostic field: ") +
                                                           hipGetErrorString(e));
   return 0;

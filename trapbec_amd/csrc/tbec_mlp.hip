@@ -245,7 +245,7 @@ int gemm(tbec_mlp *m, const float *A, long long sam, long long sak, const float 
   if (splits > 1) {
     const long long need = (long long)splits * M * N;
     if (need > m->split_elems) {
-      if (m->split) hipFree(m->split);
+      if (m->split) (void)hipFree(m->split);
       m->split = nullptr;
       MCHK(hipMalloc(&m->split, sizeof(float) * need));
       m->split_elems = need;
@@ -267,12 +267,12 @@ int gemm(tbec_mlp *m, const float *A, long long sam, long long sak, const float 
 
 int ensure_batch(tbec_mlp *m, int B) {
   if (B <= m->cap_batch) return 0;
-  for (float *p : m->H) if (p) hipFree(p);
-  for (float *p : m->Z) if (p) hipFree(p);
-  for (float *p : {m->dA, m->dB, m->Y, m->dY, m->T}) if (p) hipFree(p);
-  if (m->part) hipFree(m->part);
-  if (m->h_part) hipHostFree(m->h_part);
-  if (m->d_idx) hipFree(m->d_idx);
+  for (float *p : m->H) if (p) (void)hipFree(p);
+  for (float *p : m->Z) if (p) (void)hipFree(p);
+  for (float *p : {m->dA, m->dB, m->Y, m->dY, m->T}) if (p) (void)hipFree(p);
+  if (m->part) (void)hipFree(m->part);
+  if (m->h_part) (void)hipHostFree(m->h_part);
+  if (m->d_idx) (void)hipFree(m->d_idx);
   const int L = (int)m->dims.size() - 1;
   m->H.assign(L, nullptr);
   m->Z.assign(L, nullptr);
@@ -401,16 +401,16 @@ int tbec_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, u
 
 int tbec_mlp_destroy(tbec_mlp *m) {
   if (!m) return 0;
-  if (m->stream) hipStreamSynchronize(m->stream);
-  for (float *p : m->H) if (p) hipFree(p);
-  for (float *p : m->Z) if (p) hipFree(p);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (float *p : m->H) if (p) (void)hipFree(p);
+  for (float *p : m->Z) if (p) (void)hipFree(p);
   for (float *p : {m->params, m->grads, m->adam_m, m->adam_v, m->dA, m->dB, m->Y, m->dY, m->T,
                    m->split, m->X_all, m->T_all})
-    if (p) hipFree(p);
-  if (m->part) hipFree(m->part);
-  if (m->h_part) hipHostFree(m->h_part);
-  if (m->d_idx) hipFree(m->d_idx);
-  if (m->stream) hipStreamDestroy(m->stream);
+    if (p) (void)hipFree(p);
+  if (m->part) (void)hipFree(m->part);
+  if (m->h_part) (void)hipHostFree(m->h_part);
+  if (m->d_idx) (void)hipFree(m->d_idx);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return 0;
 }
@@ -466,8 +466,8 @@ int tbec_mlp_loss_grad(tbec_mlp *m, const float *x, const float *t, int32_t batc
 
 int tbec_mlp_set_data(tbec_mlp *m, const float *x, const float *t, int32_t n) {
   if (!m || !x || !t || n < 1) return set_error(TBEC_ERR_ARG, "bad argument");
-  if (m->X_all) hipFree(m->X_all);
-  if (m->T_all) hipFree(m->T_all);
+  if (m->X_all) (void)hipFree(m->X_all);
+  if (m->T_all) (void)hipFree(m->T_all);
   m->X_all = m->T_all = nullptr;
   MCHK(hipMalloc(&m->X_all, sizeof(float) * (size_t)n * m->n_in));
   MCHK(hipMalloc(&m->T_all, sizeof(float) * (size_t)n * m->n_out));
@@ -588,7 +588,7 @@ extern "C" int tbec_upload_coarse(tbec_handle *h, const float *coarse, int32_t c
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, hipGetErrorString(e));
   return tbec_reset_state(h);
 }

@@ -31,7 +31,7 @@ double tbec::progress_interval() {
   } while (0)
 
 void tbec_drop_graphs(tbec_handle *h) {
-  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
 }
 
@@ -70,7 +70,7 @@ static int replay_chunk(tbec_handle *h, long long n, int mode) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
-      if (g) hipGraphDestroy(g);
+      if (g) (void)hipGraphDestroy(g);
       h->cur = cur0;
       return rc;
     }
@@ -78,7 +78,7 @@ static int replay_chunk(tbec_handle *h, long long n, int mode) {
                                                             hipGetErrorString(e));
     hipGraphExec_t ex = nullptr;
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return set_error(TBEC_ERR_HIP, std::string("graph instantiate: ") +
                                                             hipGetErrorString(e));
     h->cur = cur0;  // capture recorded the parity flips; the replay performs them
@@ -259,8 +259,8 @@ int tbec_time_stages(tbec_handle *h, int nstages, double *ms_per_stage) {
   HIPCHK2(hipEventSynchronize(e1));
   float ms = 0;
   HIPCHK2(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   *ms_per_stage = ms / nstages;
   return imag ? tbec_materialize_scale(h) : 0;
 }

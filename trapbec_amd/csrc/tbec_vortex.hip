@@ -74,8 +74,8 @@ extern "C" int tbec_vortex_cores(tbec_handle *h, double dens_min, int orientatio
   unsigned long long *d_cnt = nullptr;
   const long long dcap = std::max<long long>(cap, 1);
   auto fail = [&](const char *what, hipError_t e) {
-    if (d_out) hipFree(d_out);
-    if (d_cnt) hipFree(d_cnt);
+    if (d_out) (void)hipFree(d_out);
+    if (d_cnt) (void)hipFree(d_cnt);
     return set_error(TBEC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hipError_t e = hipMalloc(&d_out, sizeof(int) * 4 * dcap);
@@ -112,8 +112,8 @@ extern "C" int tbec_vortex_cores(tbec_handle *h, double dens_min, int orientatio
     for (long long q = 0; q < nget; ++q)
       for (int c = 0; c < 4; ++c) out[4 * q + c] = v[q][c];
   }
-  hipFree(d_out);
-  hipFree(d_cnt);
+  (void)hipFree(d_out);
+  (void)hipFree(d_cnt);
   *count = (int64_t)cnt;
   return 0;
 }
