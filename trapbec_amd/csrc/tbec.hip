@@ -491,18 +491,16 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
     return fail(TBEC_ERR_ARG, "box lengths must be positive");
   if (p->tile < -1 || p->tile >= kNumTiles)
     return fail(TBEC_ERR_ARG, "tile id %d out of range [-1, %d)", p->tile, kNumTiles);
+  Layout L;
+  compute_layout(*p, L);
+  if (L.sx * L.csize > 0x7fffffffLL)     // one x-plane is one buffer record range
+    return fail(TBEC_ERR_ARG, "x-plane of %lld bytes exceeds 2^31", L.sx * L.csize);
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   tbec_handle *h = new tbec_handle();
   h->p = *p;
   h->tile = p->tile < 0 ? 2 : p->tile;
   h->seg = auto_seg(tiles_per_plane(*p, h->tile), p->nx, kAutoSegBlocks);
-  Layout L;
-  compute_layout(*p, L);
-  if (L.sx * L.csize > 0x7fffffffLL) {   // one x-plane is one buffer record range
-    delete h;
-    return fail(TBEC_ERR_ARG, "x-plane of %lld bytes exceeds 2^31", L.sx * L.csize);
-  }
   h->csize = L.csize;
   h->hx = L.hx;
   h->hy = L.hy;
