@@ -695,9 +695,9 @@ gpe_stage(const StageArgs a) {
   }
 }
 
-// Many condensates in one launch.  Work items are (instance, tile) pairs; a fixed grid
-// of workgroups walks them with a grid stride.  Per-instance static arguments live in
-// a device table; the launch supplies the stage and the buffer parity.
+// Many condensates in one launch.  Work items are (instance, tile) pairs, one per
+// workgroup.  Per-instance static arguments live in a device table; the launch supplies
+// the stage and the buffer parity.
 struct BatchInst {
   StageArgs a;          // psi_in/psi_out/w, stage fields are filled per item
   void *psi[2];
@@ -723,9 +723,13 @@ template <typename T, int TZ, int TY, int VZ, int SUM, int PHYS, bool IMAG, int 
 __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) * TY, IMAG>()))
 gpe_stage_batch(const BatchLaunch L) {
   constexpr int NT = (TZ / VZ) * TY;
-  for (int item = blockIdx.x; item < L.total_items; item += gridDim.x) {
+  // one work item per workgroup (the grid is total_items): no loop state stays live
+  // across the plane march
+  const int item = blockIdx.x;
+  if (item >= L.total_items) return;
+  {
     const int i = L.item_inst[item];
-    if (L.flags[4 * i] != 0) continue;          // converged or failed: skip (uniform)
+    if (L.flags[4 * i] != 0) return;            // converged or failed: skip (uniform)
     const BatchInst &bi = L.inst[i];
     StageArgs a = bi.a;
     a.psi_in = bi.psi[L.parity];
@@ -746,7 +750,6 @@ gpe_stage_batch(const BatchLaunch L) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
     march_tile<T, TZ, TY, VZ, SUM, PHYS, IMAG, SK>(a, item - bi.item_begin, red);
     if constexpr (IMAG) block_sum4<NT>(red, L.partials + 4LL * item);
-    __syncthreads();
   }
 }
 

@@ -1,7 +1,7 @@
 // Batched ensembles: many independent condensates of uneven grids and parameters,
 // advanced in imaginary time together.  Every RK stage of every instance is one launch
-// of gpe_stage_batch over (instance, tile) work items with a fixed grid that walks the
-// items; the per-instance finalisations (energy, convergence, renormalisation) are one
+// of gpe_stage_batch with one workgroup per (instance, tile) work item; the per-instance
+// finalisations (energy, convergence, renormalisation) are one
 // launch with a workgroup per instance.  Runs of steps are captured into a graph.
 #include <hip/hip_runtime.h>
 
@@ -195,7 +195,7 @@ int enqueue_batch_stage(tbec_batch *b, int kind) {
   L.state = b->d_state;
   L.state_stride = ST_COUNT;
   L.flags = b->d_flags;
-  const int grid = std::min(b->total_items, 4096);
+  const int grid = b->total_items;   // one workgroup per (instance, tile) work item
   hipError_t e = launch_batch(b->prec, b->tile, L, b->sum, b->rot, grid, b->stream);
   if (e != hipSuccess)
     return set_error(TBEC_ERR_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
