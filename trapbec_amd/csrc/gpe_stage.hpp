@@ -61,6 +61,9 @@ enum SumMode { SUM_REF = 0, SUM_COMP = 1, SUM_CENTRED = 2 };
 //            (src/finite_diff.f90:126-130).
 enum PhysKind { PH_HARM = 0, PH_ROT = 1, PH_GEN = 2 };
 enum StageKind { ST_RK1 = 0, ST_RK2 = 1, ST_RK3 = 2, ST_EULER = 3, ST_RHS = 4 };
+// Stage-kind template argument of the kernel instances with the kind fixed at compile
+// time: an RK stage kind, plus SK_STORED for the stored-w scheme (w rebuilt otherwise).
+constexpr int SK_STORED = 8;
 
 template <typename T> struct alignas(2 * sizeof(T)) cplx { T re, im; };
 
@@ -369,8 +372,13 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // psi_{s-1}; the host sets scale / prev_scale only for those stages.  Instances with the
   // stage kind fixed carry only the one they use (a multiply by 1 is exact, so the other
   // instances give the same results).
-  constexpr bool USE_SC = IMAG && (SK < 0 || SK == ST_RK1);
-  constexpr bool USE_PSC = IMAG && (SK < 0 || SK == ST_RK2);
+  // SK >= 0: the stage kind is fixed at compile time (KIND; with SK_STORED the stored-w
+  // scheme, else w rebuilt), which removes the run-time stage branches from the plane loop
+  constexpr bool FIX = SK >= 0;
+  constexpr bool STO = FIX && (SK & SK_STORED) != 0;
+  constexpr int KIND = FIX ? (SK & 7) : -1;
+  constexpr bool USE_SC = IMAG && (!FIX || KIND == ST_RK1);
+  constexpr bool USE_PSC = IMAG && (!FIX || KIND == ST_RK2);
   T sc = T(1), psc = T(1);
   T c0 = T(a.c0);
   if constexpr (IMAG) {
@@ -447,10 +455,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   if (hl) ldh(rs(pin + (long long)xs * sx), hoffb, hb[0]);
   // w of the first plane (stages that read w); later planes are prefetched one ahead
   C wq[2][VZ];
-  // SK >= 0: the stage kind is fixed at compile time (real time, w rebuilt), which removes
-  // the run-time stage branches from the plane loop
-  constexpr bool FIX = SK >= 0;
-  if (stage_reads_w(FIX ? SK : a.stage))
+  if (stage_reads_w(FIX ? KIND : a.stage))
     BufIO<T, VZ>::template ld<kPointwiseLoadCpol>(rs(p_w), ownb, wq[0]);
   // tabulated potential (PH_GEN): own points of plane x, prefetched one plane ahead
   constexpr bool VT = PHYS == PH_GEN;
@@ -469,9 +474,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const T gnl = T(a.gnl), om = T(a.om);
   const T cre = T(a.cre), cim = T(a.cim), A = T(a.A), B = T(a.B);
   const T wrec = T(a.wrec);
-  const bool rebuild = FIX ? (SK == ST_RK2 || SK == ST_RK3) : a.wrec != 0.0;
-  const bool store_w = FIX ? false : a.store_w != 0;
-  const int stage = FIX ? SK : a.stage;
+  const bool rebuild = FIX ? (!STO && (KIND == ST_RK2 || KIND == ST_RK3)) : a.wrec != 0.0;
+  const bool store_w = FIX ? (STO && (KIND == ST_RK1 || KIND == ST_RK2)) : a.store_w != 0;
+  const int stage = FIX ? KIND : a.stage;
   const T yc = T(a.y0 + a.hy * y);
   T zc2[VZ];
 #pragma unroll

@@ -55,13 +55,13 @@ TBEC_DECL_BLAUNCHER(64, 4)
 hipError_t launch_batch(int prec, int tile, const BatchLaunch &L, int sum, bool rot, int grid,
                         hipStream_t s);
 
-// Stages with w rebuilt in the default summation of each precision (centred FP32,
-// reference FP64) also come as instances with the stage kind fixed at compile time, in
-// real and imaginary time and in the batch kernel: the plane loop then carries no stage
+// Stages in the default summation of each precision (centred FP32, reference FP64) also
+// come as instances with the stage kind fixed at compile time: with w rebuilt in real and
+// imaginary time and in the batch kernel, and with w stored in real time: the plane loop then carries no stage
 // branches and the imaginary-time variants spill less (measured: real time at 512^3
 // FP32 -8.5% per step, FP64 rotating -4%; imaginary time at 256^3 rotating FP32 -12%,
-// FP64 -18%; a batch of 12 x 96^3 -21% / -16%).  Stored w and the other FP32 sums use the
-// run-time stage kind.
+// FP64 -18%; a batch of 12 x 96^3 -21% / -16%; stored w at 512^3 FP32 -2%, FP64 0%).  The
+// other FP32 sums and imaginary time with stored w use the run-time stage kind.
 #ifndef TBEC_STAGE_SPECIALISE
 #define TBEC_STAGE_SPECIALISE 1
 #endif
@@ -108,6 +108,22 @@ static void launch_phys(const StageArgs &a, bool imag, hipStream_t s) {
     }
     if (rebuilt && a.stage == ST_RK3) {
       hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK3>), grid, block, 0, s, a);
+      return;
+    }
+    // the stored-w scheme: stages 1 and 2 write w, stages 2 and 3 read it
+    if (a.stage == ST_RK1 && a.store_w && a.wrec == 0.0) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK1 | SK_STORED>), grid,
+                         block, 0, s, a);
+      return;
+    }
+    if (a.stage == ST_RK2 && a.store_w && a.wrec == 0.0) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK2 | SK_STORED>), grid,
+                         block, 0, s, a);
+      return;
+    }
+    if (a.stage == ST_RK3 && !a.store_w && a.wrec == 0.0) {
+      hipLaunchKernelGGL((gpe_stage<T, TZ, TY, VZ, S, PH, false, ST_RK3 | SK_STORED>), grid,
+                         block, 0, s, a);
       return;
     }
   }
