@@ -4,7 +4,6 @@ fixtures) and the CPU oracle on the same seeded inputs.
 Tolerances: FP64 psi within 1e-12 relative max-norm (after up to 1000 steps); FP32
 within the stated FP32 bounds.  Every case prints its name and shape before running."""
 import glob
-import math
 import os
 
 import numpy as np

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 from scipy.ndimage import map_coordinates
 
-from trapbec_amd import initial, tbec
+from trapbec_amd import tbec
 
 pytestmark = pytest.mark.gpu
 
