@@ -543,6 +543,11 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
       for (int k = 0; k < VZ + 8; ++k) zr[k] = row[k];
     }
     C out[VZ], wo[VZ];
+    // FP32 instances of the last imaginary-time stage sum the VZ values of |psi|^2 of a
+    // plane in FP32 and add that to the FP64 accumulator once per plane: the norm only
+    // sets the renormalisation scale of an FP32 psi (the energy sums stay FP64 per point)
+    constexpr bool PN32 = IMAG && sizeof(T) == 4 && KIND == ST_RK3;
+    T pn = T(0);
 #pragma unroll
     for (int v = 0; v < VZ; ++v) {
       const int col = 4 + tz * VZ + v;
@@ -622,12 +627,16 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
             red[2] += (double)n2 * (double)n2;
             red[3] += (double)rr * (double)rr + (double)ri * (double)ri;
           } else if (stage == ST_RK3 || stage == ST_EULER) {
-            red[0] += (double)out[v].re * (double)out[v].re +
-                      (double)out[v].im * (double)out[v].im;
+            if constexpr (PN32)
+              pn += out[v].re * out[v].re + out[v].im * out[v].im;
+            else
+              red[0] += (double)out[v].re * (double)out[v].re +
+                        (double)out[v].im * (double)out[v].im;
           }
         }
       }
     }
+    if constexpr (PN32) red[0] += (double)pn;
     // stores (interior points only; padding stays zero)
     if (yin) {
       C *po = p_out;   // uniform plane bases
