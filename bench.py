@@ -62,6 +62,9 @@ def parse():
                     help="time the kernels beyond the headline path at n^3 FP32 (stage "
                          "variants per physics, potential tables, reference states, "
                          "imprint, output diagnostics) and print one JSON line")
+    ap.add_argument("--imag-probe", action="store_true",
+                    help="imaginary-time solve steps of the rotating 256^3 ground-state "
+                         "problem (FP32 and FP64), for kernel traces")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: 3 FP32 stages at n^3 plus a plain "
                          "device copy of one psi-sized buffer (the counter calibration)")
@@ -239,6 +242,31 @@ def pmc_probe(args):
                       "stage_bytes_moved_avg": n ** 3 * 64 / 3}), flush=True)
 
 
+def imag_probe(args):
+    """Imaginary-time solve steps of the rotating 256^3 ground-state problem
+    (initial.config_c3: Omega = 0.7, dtau = 1e-3), FP32 then FP64, each with the tile
+    recorded for it: 200 warm-up steps, then 1000 steps checked every 200 (a convergence
+    tolerance of 1e-30 never stops them).  Prints ms per step on the host clock."""
+    import torch
+    from trapbec_amd import initial, tbec
+    torch.cuda.synchronize()
+    pr, psi0 = initial.config_c3(n=256)
+    out = {"imag_probe": "256^3 rotating imaginary time"}
+    for prec, (tile, seg, remap) in ((32, (2, 32, 1)), (64, (0, 64, 0))):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.set_tuning(tile, seg)
+            c.set_xcd_remap(remap)
+            c.upload(psi0)
+            c.imag_solve(200, 1e-30, 1.0, 200)
+            c.sync()
+            t0 = time.perf_counter()
+            c.imag_solve(1000, 1e-30, 1.0, 200)
+            c.sync()
+            out[f"fp{prec}_ms_per_step"] = round((time.perf_counter() - t0) * 1e3 / 1000, 5)
+            out[f"fp{prec}_tile"] = [tile, seg, remap]
+    print(json.dumps(out), flush=True)
+
+
 def extensions_probe(args):
     """ms per step of the stage variants each physics choice selects, and the one-off
     kernels (potential tables, the mobius Trap at (n/2)^3, reference states, the vortex
@@ -343,6 +371,9 @@ def main():
         return
     if args.extensions:
         extensions_probe(args)
+        return
+    if args.imag_probe:
+        imag_probe(args)
         return
     pr, spec = initial.ic_spec("m", n)
     res = {}
