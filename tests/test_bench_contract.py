@@ -55,3 +55,21 @@ def test_bench_json_line_at_64():
                                           abs=2e-4)
     assert sc["frac"] == pytest.approx(out["value"] / sc["value"], rel=1e-3)
     assert out["fp64"]["value"] > 0
+
+
+@pytest.mark.gpu
+def test_bench_short_run_agrees_with_long_run():
+    """The 16-step graphs are captured before the timed region whatever --steps and
+    --warmup are: a 20-step run after 5 warm-up steps (odd parity, one graph replay and 4
+    eager steps) is not slower than a 200-step run by more than 3% at 512^3."""
+    ms = {}
+    for steps, warm in ((20, 5), (200, 30)):
+        print(f"\n[case] bench.py --n 512 --only f32 --headline-only --steps {steps} "
+              f"--warmup {warm}", flush=True)
+        r = run_bench("--n", "512", "--only", "f32", "--headline-only", "--no-cpu",
+                      "--steps", str(steps), "--warmup", str(warm))
+        assert r.returncode == 0, r.stderr[-3000:]
+        ms[steps] = json.loads(r.stdout.strip().splitlines()[-1])["ms_per_step"]
+        print(f"[case] {steps} steps: {ms[steps]} ms/step", flush=True)
+    assert ms[20] <= 1.03 * ms[200], ms
+    assert ms[20] >= 0.85 * ms[200], ms

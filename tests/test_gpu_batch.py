@@ -34,6 +34,22 @@ def test_batch_equals_single_solves(prec):
         assert res[i]["energy"] == r["energy"]
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_upload_after_solve_round_trips(prec):
+    """An upload after a solve replaces the instance's state as given: the download
+    returns it bit for bit (no renormalisation scale left over from the solve)."""
+    probs = _problems()[:3]
+    print(f"\n[case] batch re-upload after solve prec={prec}", flush=True)
+    with tbec.Batch([p for p, _ in probs], prec=prec) as b:
+        for i, (_, psi0) in enumerate(probs):
+            b.upload(i, psi0)
+        b.imag_solve(100, 1e-30, 1.0, 20)
+        x = (2.0 * probs[1][1]).astype(b.cdt)
+        b.upload(1, x)
+        got = b.download(1)
+    assert np.array_equal(got, x)
+
+
 def test_batch_rejects_mixed_precision():
     probs = _problems()[:2]
     with pytest.raises(tbec.TbecError):

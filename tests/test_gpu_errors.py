@@ -1,6 +1,6 @@
 """Error paths through the C ABI: a non-finite psi is reported as TBEC_ERR_NONFINITE by
-the FP64 diagnostics, the imaginary-time solve and the batch solve, and the handle works
-again after a finite upload.  NaN is ordinary data for the kernels (nothing faults); the
+the FP64 diagnostics, the imaginary-time solve, step-limited imaginary-time steps and the
+batch solve, and the handle works again after a finite upload.  NaN is ordinary data for the kernels (nothing faults); the
 host reads the FP64 reductions and turns a non-finite norm or energy into the error."""
 import numpy as np
 import pytest
@@ -61,3 +61,21 @@ def test_batch_solve_reports_nonfinite():
         with pytest.raises(tbec.TbecError) as e:
             b.imag_solve(40, 1e-10, 1.0, 10)
         assert e.value.code == ERR_NONFINITE
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_imag_steps_report_nonfinite_and_recover(prec):
+    """tbec_step(TBEC_IMAG_RK3) reads the finalisers' flag: a NaN norm is an error, not a
+    silently frozen psi (40 steps: two replayed 16-step graphs and 8 eager steps)."""
+    print(f"\n[case] imag steps NaN prec={prec}", flush=True)
+    good, bad = states()
+    pr = initial.Problem(SHAPE, BOX, OM, G, 0.3, -2e-3j)
+    with tbec.Condensate.from_problem(pr, prec=prec) as c:
+        c.upload(bad)
+        with pytest.raises(tbec.TbecError) as e:
+            c.step(40, tbec.IMAG_RK3)
+        assert e.value.code == ERR_NONFINITE
+        c.upload(good)
+        c.step(40, tbec.IMAG_RK3)
+        d = c.diagnose()
+        assert np.isfinite(d["energy"]) and d["norm"] == pytest.approx(1.0, rel=1e-5)

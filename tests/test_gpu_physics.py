@@ -208,6 +208,26 @@ def test_switch_times_long_run_with_graphs():
     assert relmax(got, s.interior) < 1e-12
 
 
+def test_switch_after_replayed_graphs():
+    """One 80-step call whose switch-offs fall after 32 and 48 replayed steps: the
+    captured graphs of the first run are dropped while their replays may still be in
+    flight (the library drains the stream first), and the result matches the oracle."""
+    shape, box = (16, 16, 16), (16.0, 16.0, 16.0)
+    announce("switch times after replayed graphs, 80 steps")
+    f = np.load(os.path.join(GOLDEN, "pot_harm_switch_f64.npz"))
+    harm = tuple(float(h) for h in f["harm_w"])
+    kw = dict(potential_off=0.0325, t_spindown=0.0485)
+    s = oracle.Stepper(oracle.params(shape, box, harm, omega_z=-1.4, **kw), f["psi0"])
+    s.rk3(80)
+    with tbec.Condensate(shape, box, harm, omega_z=-1.4, dt=1e-3, prec=64) as c:
+        c.set_physics("harmonic", **kw)
+        c.upload(f["psi0"])
+        c.step(80)
+        got = c.download()
+        assert c.t == pytest.approx(s.t, rel=1e-15)
+    assert relmax(got, s.interior) < 1e-12
+
+
 def test_box_trap_imaginary_time_vs_oracle():
     """Renormalised imaginary-time RK3 in the box trap (tabulated potential, evaluated at
     the handle's time, t = 0 here as in the reference's imaginary loop) against the oracle."""

@@ -176,7 +176,11 @@ int build_table(tbec_batch *b) {
   return 0;
 }
 
+// A replay may still be in flight: drain the stream before destroying its graphs.
 void drop_graphs(tbec_batch *b) {
+  bool any = false;
+  for (auto &g : b->graph) any = any || g;
+  if (any && b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto &g : b->graph)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   b->graph_steps = 0;
@@ -386,6 +390,11 @@ int tbec_batch_upload_psi(tbec_batch *b, int32_t i, const void *psi_host) {
                        (const cplx<double> *)b->stage, (cplx<double> *)dst, q.nx, q.ny, q.nz,
                        L.sy, L.sx, L.base);
   BCHK(hipGetLastError());
+  // the uploaded state is the instance's psi as given: no pending renormalisation
+  // (tbec_batch_download_psi multiplies by the scale a solve leaves behind)
+  const double one = 1.0;
+  BCHK(hipMemcpyAsync(b->d_state + (size_t)i * ST_COUNT + ST_SCALE, &one, sizeof(double),
+                      hipMemcpyHostToDevice, b->stream));
   BCHK(hipStreamSynchronize(b->stream));
   b->cur[i] = b->parity;
   return 0;

@@ -30,7 +30,12 @@ double tbec::progress_interval() {
       return set_error(TBEC_ERR_HIP, std::string(#x " failed: ") + hipGetErrorString(e_)); \
   } while (0)
 
+// Graph replays are asynchronous: a replay can still be in flight when a caller drops
+// the graphs (a physics change mid-run, tuning, destroy), so the stream is drained
+// before any executable graph is destroyed.
 void tbec_drop_graphs(tbec_handle *h) {
+  if (h->graphs.empty()) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
 }
@@ -134,6 +139,14 @@ int tbec_step(tbec_handle *h, int64_t nsteps, int mode, double *t_inout) {
     rc = run_steps(h, nsteps, mode, nsteps >= kChunk ? kChunk : 0);
     if (rc) return rc;
     if (t_inout) *t_inout = h->t_now;
+    // the finalisers flag a non-finite norm or energy (and every later stage skips)
+    HIPCHK2(hipMemcpyAsync(h->hflag, h->dflag, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                           h->stream));
+    HIPCHK2(hipStreamSynchronize(h->stream));
+    if (h->hflag[0] == 2) {
+      h->scale_pending = false;
+      return set_error(TBEC_ERR_NONFINITE, "non-finite norm or energy in imaginary time");
+    }
     return tbec_materialize_scale(h);
   }
   int rc = tbec_materialize_scale(h);
