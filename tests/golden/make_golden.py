@@ -153,6 +153,45 @@ def case_pot(name, prec, potential, shape=(24, 24, 24), box=(16.0, 16.0, 16.0), 
     return r
 
 
+def parse_shot_lines(stdout: str) -> np.ndarray:
+    """The per-shot diagnostics line of program run (src/run.f90:59-63): list-directed
+    `itime/shots, t, maxPsi, minPsi, meanPsi, intPsi2`, one row per shot.  List-directed
+    output wraps long records over lines, so the values after the restart message are
+    read as one token stream, six per row."""
+    tail = stdout.split("restart, t=", 1)[1].split("\n", 1)[1]
+    vals = [float(v) for v in tail.split()]
+    assert len(vals) % 6 == 0, tail
+    return np.array(vals, dtype=np.float64).reshape(-1, 6)
+
+
+def case_info(name, prec, nprocs=1, shape=(12, 10, 8), box=(12.0, 10.0, 8.0), steps=20,
+              shots=5, seed=23, keep_var=False):
+    """Psi_info / Psi_norm (src/diagnostic.f90:7-25) as the reference prints them every
+    `shots` steps (src/run.f90:59-63): real-time RK3 from a seeded state on a ragged grid;
+    stored: psi0, the printed rows and the state of the first shot (var0001.dat).  keep_var also stores the raw
+    per-rank snapshot records (src/output.f90:71-78) for the gather test."""
+    harm = tuple(L * L for L in box)
+    d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
+                       Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=shots,
+                       harm_wx=harm[0], harm_wy=harm[1], harm_wz=harm[2])
+    psi0 = small_ic(shape, box, seed).astype(np.complex128 if prec == 64 else np.complex64)
+    r = refbuild.run(d, psi0, prec=prec, nprocs=nprocs)
+    rows = parse_shot_lines(r["stdout"])
+    assert rows.shape == (steps // shots, 6), (rows.shape, r["stdout"][-2000:])
+    extra = {}
+    if keep_var:
+        for rk in range(nprocs):
+            for shot in range(1, steps // shots + 1):
+                fn = os.path.join(d, "data", f"proc{rk:03d}", f"var{shot:04d}.dat")
+                with open(fn, "rb") as fh:
+                    extra[f"raw_proc{rk:03d}_var{shot:04d}"] = np.frombuffer(fh.read(),
+                                                                              np.uint8)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), psi0=psi0, psi=r["psi"],
+                        rows=rows, steps=steps, shots=shots, prec=prec, box=np.array(box),
+                        harm_w=np.array(harm), nprocs=nprocs, dt=1e-3, **extra)
+    return r
+
+
 def case_ic(name, prec, init_cond, shape=(20, 16, 12), box=(16.0, 14.0, 12.0),
             harm=(4.0, 4.0, 4.0), potential="harmonic", winding_number=1, ring_rad=4.0,
             add_ring=False, add_dipole=False, add_pcurrent=False, phase_imprint_on=False):
@@ -234,6 +273,11 @@ CASES = {
                                                   omega_z=-0.8, periodicz=True),
     "rk3_f64_16_solid_np4": lambda: case_small("rk3_f64_16_solid_np4", 64, omega_z=-1.4,
                                                boundaries="solid", nprocs=4),
+    "info_f64_ragged": lambda: case_info("info_f64_ragged", 64),
+    "info_f32_ragged": lambda: case_info("info_f32_ragged", 32),
+    "info_f64_ragged_np4": lambda: case_info("info_f64_ragged_np4", 64, nprocs=4,
+                                             keep_var=True),
+    "info_f32_ragged_np4": lambda: case_info("info_f32_ragged_np4", 32, nprocs=4),
     "euler_f64_16": case_euler,
     "diag_f64_16": lambda: case_diag("diag_f64_16", 64),
     "diag_f32_16": lambda: case_diag("diag_f32_16", 32),

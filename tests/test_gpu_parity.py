@@ -72,12 +72,20 @@ def test_c1_full_size_vs_reference(stored):
         c.upload(psi0)
         c.step(int(f["steps"]))
         got = c.download()
+        d = c.diagnose()
     assert relmax(got[::4, ::4, ::4], f["sub"]) < 1e-12
     assert relmax(got[:, :, 32], f["plane"]) < 1e-12
     kp = pr.kernel_params()
-    s = oracle.Stepper(oracle.params(pr.shape, pr.box, kp["harm_w"], dt=1e-3), psi0)
+    s = oracle.Stepper(oracle.params(pr.shape, pr.box, kp["harm_w"], c0=kp["c0"],
+                                     gnl=kp["gnl"], dt=1e-3), psi0)
     s.rk3(int(f["steps"]))
     assert relmax(got, s.interior) < 1e-12
+    # norm and energy after 1000 steps: the reference's own sum|psi|^2, and the FP64
+    # energy functional of the oracle's state
+    assert d["int_psi2"] == pytest.approx(float(f["norm2"]), rel=1e-12)
+    e = s.energy()
+    assert d["norm"] == pytest.approx(e["norm"], rel=1e-12)
+    assert d["energy"] == pytest.approx(e["energy"], rel=1e-12)
 
 
 def _rand_psi(shape, seed, scale=1.0):
@@ -152,6 +160,32 @@ def test_diagnostics_vs_oracle():
     assert d["min_abs"] == pytest.approx(info[1], abs=1e-14)
     assert d["mean_abs"] == pytest.approx(info[2], rel=1e-12)
     assert d["int_psi2"] == pytest.approx(info[3], rel=1e-12)
+
+
+@pytest.mark.parametrize("name", ["info_f64_ragged", "info_f32_ragged",
+                                  "info_f64_ragged_np4", "info_f32_ragged_np4"])
+def test_diagnose_vs_reference_printout(name):
+    """tbec_diagnose's max|psi|, min|psi|, mean|psi| and sum|psi|^2 (FP64 reductions)
+    against the values the reference itself prints every `shots` steps
+    (src/run.f90:59-63, src/diagnostic.f90:7-25), on a ragged 12x10x8 grid run on 1 and
+    4 ranks.  The reference reduces FP32 runs in FP32, so FP32 sums agree to ~1e-6."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    prec, shots = int(f["prec"]), int(f["shots"])
+    announce(f"{name} prec={prec} shots={shots}")
+    box = tuple(float(b) for b in f["box"])
+    with tbec.Condensate(f["psi0"].shape, box, tuple(float(h) for h in f["harm_w"]),
+                         dt=float(f["dt"]), prec=prec) as c:
+        c.upload(f["psi0"])
+        for row in f["rows"]:
+            c.step(shots)
+            d = c.diagnose()
+            rt_max, rt_sum = (1e-13, 1e-12) if prec == 64 else (2e-6, 4e-6)
+            assert c.t == pytest.approx(row[1], rel=1e-14 if prec == 64 else 1e-7)
+            assert d["max_abs"] == pytest.approx(row[2], rel=rt_max)
+            # min|psi| is small: its error is that of psi, relative to max|psi|
+            assert d["min_abs"] == pytest.approx(row[3], abs=rt_max * row[2])
+            assert d["mean_abs"] == pytest.approx(row[4], rel=rt_sum)
+            assert d["int_psi2"] == pytest.approx(row[5], rel=rt_sum)
 
 
 @pytest.mark.parametrize("stored", [False, True])

@@ -606,3 +606,57 @@ def test_add_noise_range():
     d = s.interior.reshape(-1)
     assert np.abs(d.real).max() <= 0.01 and np.abs(d.imag).max() <= 0.01
     assert np.allclose(d.real, 0.01 * (2 * u[0::2] - 1), rtol=0, atol=1e-17)
+
+
+# ---- Psi_info / Psi_norm (src/diagnostic.f90:7-25) ------------------------------------
+INFO = ["info_f64_ragged", "info_f32_ragged", "info_f64_ragged_np4", "info_f32_ragged_np4"]
+
+
+@pytest.mark.parametrize("name", INFO)
+def test_psi_info_matches_reference_printout(name):
+    """The oracle's max|psi|, min|psi|, mean|psi| and sum|psi|^2 equal the values the
+    reference prints every `shots` steps (src/run.f90:59-63), at the print precision of
+    list-directed output (17 digits FP64, 9 FP32), on a ragged 12x10x8 grid, run on 1 and
+    on 4 ranks (the 4-rank means and sums are reduced over ranks and divided by the
+    global point count, src/diagnostic.f90:16-17)."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"))
+    prec, shots = int(f["prec"]), int(f["shots"])
+    box = tuple(float(b) for b in f["box"])
+    harm = tuple(float(h) for h in f["harm_w"])
+    s = oracle.Stepper(oracle.params(f["psi0"].shape, box, harm, dt=1e-3), f["psi0"],
+                       prec=prec)
+    # FP64: print rounding and summation order (960 terms); FP32: the reference sums |psi|
+    # in FP32 in its own order, the oracle in long double
+    rt_max, rt_sum = (1e-15, 1e-13) if prec == 64 else (1e-7, 2e-6)
+    for i, row in enumerate(f["rows"]):
+        s.rk3(shots)
+        if i == 0:
+            # the state of the first shot is the reference's own snapshot, bit for bit
+            assert np.array_equal(s.interior, f["psi"])
+        info = s.info()
+        assert row[1] == pytest.approx(s.t, rel=1e-15 if prec == 64 else 1e-7)
+        assert info[0] == pytest.approx(row[2], rel=rt_max)
+        assert info[1] == pytest.approx(row[3], rel=rt_max)
+        assert info[2] == pytest.approx(row[4], rel=rt_sum)
+        assert info[3] == pytest.approx(row[5], rel=rt_sum)
+
+
+def test_psi_info_brute_force():
+    """orc_psi_info against a direct evaluation on a ragged grid with zeros and a wide
+    dynamic range: max and min of |psi|, the mean over all Nx*Ny*Nz points and the
+    exactly rounded sum of |psi|^2 (math.fsum)."""
+    rng = np.random.default_rng(5)
+    shape = (7, 5, 9)
+    psi = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * \
+        np.exp(rng.uniform(-6, 3, shape))
+    psi[2, 3, 4] = 0.0
+    p = oracle.params(shape, (7.0, 5.0, 9.0), (1.0, 1.0, 1.0))
+    info = oracle.Stepper(p, psi).info()
+    a = np.abs(psi)
+    assert info[0] == pytest.approx(a.max(), rel=3e-16) and info[1] == 0.0
+    assert info[2] == pytest.approx(math.fsum(a.ravel()) / a.size, rel=1e-15)
+    assert info[3] == pytest.approx(math.fsum((a * a).ravel()), rel=1e-15)
+    # planted mistakes the checks above tell apart: the mean over a quarter of the points
+    # (a per-rank count) and sum|psi| in place of sum|psi|^2
+    assert abs(info[2] - 4 * info[2]) > 1e-3 * info[2]
+    assert abs(info[3] - math.fsum(a.ravel())) > 1e-3 * info[3]
