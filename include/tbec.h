@@ -117,6 +117,17 @@ int tbec_rhs(tbec_handle *h, void *rhs_host);
  * which makes it the norm of that pass's rhs (no cancellation near a ground state). */
 int tbec_diagnose(tbec_handle *h, tbec_diag *out);
 
+/* tbec_diagnose that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The reduction partials live in a static
+ * device array of 32768 four-double slots (1 MiB per GPU, held until the process ends)
+ * instead of the handle's on-demand buffer; results equal tbec_diagnose's bit for bit.
+ * TBEC_ERR_ARG when the grid, tile and segment need more slots (one per workgroup of a
+ * stage pass, at most 8192 for Psi_info); TBEC_ERR_STATE when a tabulated potential of
+ * the current time is not built yet.  Every call on one GPU shares the array: calls that
+ * overlap on one GPU (other handles, other threads) race and must be serialised. */
+int tbec_diagnose_noalloc(tbec_handle *h, tbec_diag *out);
+
 /* Imaginary-time ground state: TBEC_IMAG_RK3 steps with dtau = -dt_im until
  * |E_n - E_{n-1}| < tol*|E_n| has held for 8 consecutive steps, or max_steps;
  * renormalises to `norm` every step.  The host checks convergence every check_every

@@ -397,7 +397,7 @@ int tbec_reserve_partials(tbec_handle *h, int nb) {
 }
 
 // Enqueue one stage.  kind: StageKind; imag: use the imaginary-time device state.
-int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
+int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce, double *part) {
   StageArgs a;
   const double dr = h->dt_re, di = h->dt_im;
   // eye*0.5*dt (src/run.f90:47)
@@ -410,9 +410,12 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
   int nb = fill_args(h, a, kind, h->cur, A, B, cre, cim);
   const bool want_red = imag || reduce;
   if (want_red) {
-    int rc = tbec_reserve_partials(h, nb);
-    if (rc) return rc;
-    a.partials = h->partials;
+    if (!part) {
+      int rc = tbec_reserve_partials(h, nb);
+      if (rc) return rc;
+      part = h->partials;
+    }
+    a.partials = part;
   }
   if (imag) {
     a.c0dev = h->dstate + ST_C0EFF;
@@ -449,10 +452,10 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce) {
     return fail(TBEC_ERR_HIP, "stage launch failed: %s", hipGetErrorString(e));
   if (imag) {
     if (kind == ST_RK1) {
-      hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb,
+      hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, part, nb,
                          h->dstate, h->dflag, 1);
     } else if (kind == ST_RK3) {
-      hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(256), 0, h->stream, h->partials, nb,
+      hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(256), 0, h->stream, part, nb,
                          h->dstate, h->dflag);
     }
     HIPCHK(hipGetLastError());
@@ -499,6 +502,7 @@ int tbec_create(const tbec_params *p, tbec_handle **out) {
   HIPCHK(hipGetDevice(&dev));
   tbec_handle *h = new tbec_handle();
   h->p = *p;
+  h->device = dev;
   h->tile = p->tile < 0 ? 2 : p->tile;
   h->seg = auto_seg(tiles_per_plane(*p, h->tile), p->nx, kAutoSegBlocks);
   h->csize = L.csize;
@@ -654,53 +658,7 @@ int tbec_rhs(tbec_handle *h, void *rhs_host) {
 
 int tbec_diagnose(tbec_handle *h, tbec_diag *out) {
   if (!h || !out) return fail(TBEC_ERR_ARG, "null argument");
-  // rhs-only pass with reductions (psi read with the pending scale), then Psi_info
-  int rc = prepare_physics(h, h->t_now);
-  if (!rc) rc = enqueue_stage(h, ST_RHS, false, true);
-  if (rc) return rc;
-  const int nb_rhs = fill_args_count(h);
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,
-                     h->dstate, h->dflag, 0);
-  HIPCHK(hipGetLastError());
-  // second pass for the GP residual, with c0 = 2*mu of the first
-  hipLaunchKernelGGL(k_set_resid_c0, dim3(1), dim3(64), 0, h->stream, h->dstate);
-  h->resid_pass = true;
-  rc = enqueue_stage(h, ST_RHS, false, true);
-  h->resid_pass = false;
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(256), 0, h->stream, h->partials, nb_rhs,
-                     h->dstate);
-  HIPCHK(hipGetLastError());
-  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
-  const int nbi = grid_for(n);
-  rc = tbec_reserve_partials(h, nbi);
-  if (rc) return rc;
-  if (h->p.prec == TBEC_F32)
-    hipLaunchKernelGGL(k_info<float>, dim3(nbi), dim3(256), 0, h->stream,
-                       (const cplx<float> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
-                       h->sx, h->base, h->dstate + ST_SCALE, h->partials);
-  else
-    hipLaunchKernelGGL(k_info<double>, dim3(nbi), dim3(256), 0, h->stream,
-                       (const cplx<double> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
-                       h->sx, h->base, h->dstate + ST_SCALE, h->partials);
-  hipLaunchKernelGGL(k_fin_info, dim3(1), dim3(256), 0, h->stream, h->partials, nbi,
-                     h->dstate);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h->hstate, h->dstate, sizeof(double) * ST_COUNT,
-                        hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  const double *s = h->hstate;
-  out->norm = s[ST_N];
-  out->energy = s[ST_E];
-  out->mu = s[ST_MU];
-  out->residual = s[ST_RES];
-  out->max_abs = s[ST_MAXA];
-  out->min_abs = s[ST_MINA];
-  out->mean_abs = s[ST_SUMA] / ((double)h->p.nx * h->p.ny * h->p.nz);
-  out->int_psi2 = s[ST_SUMA2];
-  if (!std::isfinite(out->energy) || !std::isfinite(out->norm))
-    return fail(TBEC_ERR_NONFINITE, "non-finite norm or energy in diagnostics");
-  return 0;
+  return diagnose_with(h, out, nullptr, 0);
 }
 
 int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
@@ -761,6 +719,70 @@ int fill_args_count(tbec_handle *h) {
   StageArgs a;
   return fill_args(h, a, ST_RHS, h->cur, 0, 0, 0, 0);
 }
+
+namespace tbec {
+
+long long diagnose_partial_slots(tbec_handle *h) {
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  return std::max<long long>(fill_args_count(h), grid_for(n));
+}
+
+int diagnose_with(tbec_handle *h, tbec_diag *out, double *part, long long part_slots) {
+  // rhs-only pass with reductions (psi read with the pending scale), then Psi_info
+  const long long n = (long long)h->p.nx * h->p.ny * h->p.nz;
+  const int nbi = grid_for(n);
+  if (!part) {
+    int rc = tbec_reserve_partials(h, (int)diagnose_partial_slots(h));
+    if (rc) return rc;
+    part = h->partials;
+  } else if (diagnose_partial_slots(h) > part_slots) {
+    return fail(TBEC_ERR_ARG, "diagnostics need %lld reduction slots, the buffer holds %lld",
+                diagnose_partial_slots(h), part_slots);
+  }
+  int rc = prepare_physics(h, h->t_now);
+  if (!rc) rc = enqueue_stage(h, ST_RHS, false, true, part);
+  if (rc) return rc;
+  const int nb_rhs = fill_args_count(h);
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(256), 0, h->stream, part, nb_rhs,
+                     h->dstate, h->dflag, 0);
+  HIPCHK(hipGetLastError());
+  // second pass for the GP residual, with c0 = 2*mu of the first
+  hipLaunchKernelGGL(k_set_resid_c0, dim3(1), dim3(64), 0, h->stream, h->dstate);
+  h->resid_pass = true;
+  rc = enqueue_stage(h, ST_RHS, false, true, part);
+  h->resid_pass = false;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fin_resid, dim3(1), dim3(256), 0, h->stream, part, nb_rhs,
+                     h->dstate);
+  HIPCHK(hipGetLastError());
+  if (h->p.prec == TBEC_F32)
+    hipLaunchKernelGGL(k_info<float>, dim3(nbi), dim3(256), 0, h->stream,
+                       (const cplx<float> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
+                       h->sx, h->base, h->dstate + ST_SCALE, part);
+  else
+    hipLaunchKernelGGL(k_info<double>, dim3(nbi), dim3(256), 0, h->stream,
+                       (const cplx<double> *)h->psi[h->cur], h->p.nx, h->p.ny, h->p.nz, h->sy,
+                       h->sx, h->base, h->dstate + ST_SCALE, part);
+  hipLaunchKernelGGL(k_fin_info, dim3(1), dim3(256), 0, h->stream, part, nbi, h->dstate);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->hstate, h->dstate, sizeof(double) * ST_COUNT,
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const double *s = h->hstate;
+  out->norm = s[ST_N];
+  out->energy = s[ST_E];
+  out->mu = s[ST_MU];
+  out->residual = s[ST_RES];
+  out->max_abs = s[ST_MAXA];
+  out->min_abs = s[ST_MINA];
+  out->mean_abs = s[ST_SUMA] / ((double)h->p.nx * h->p.ny * h->p.nz);
+  out->int_psi2 = s[ST_SUMA2];
+  if (!std::isfinite(out->energy) || !std::isfinite(out->norm))
+    return fail(TBEC_ERR_NONFINITE, "non-finite norm or energy in diagnostics");
+  return 0;
+}
+
+}  // namespace tbec
 
 int tbec_reset_state(tbec_handle *h) {
   double *s = h->hstate;

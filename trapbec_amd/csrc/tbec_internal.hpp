@@ -78,6 +78,7 @@ double progress_interval();
 
 struct tbec_handle {
   tbec_params p{};
+  int device = 0;                       // HIP device current at tbec_create
   int tile = 0, seg = 64;
   int remap = 1;                        // XCD-aware tile order
   int w_stored = 0;                     // 1: RK3 stores w (10 transfers), 0: rebuilt (8)
@@ -119,7 +120,16 @@ struct tbec_handle {
 };
 
 namespace tbec {
-int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce);
+// part: the reduction partials of the stage (4 doubles per workgroup); null = the
+// handle's own buffer, sized (allocated) on demand
+int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce, double *part = nullptr);
+// tbec_diagnose with its reduction partials in `part` (null = the handle's own buffer,
+// allocated on demand); `part_slots` is the number of 4-double slots `part` holds
+int diagnose_with(tbec_handle *h, tbec_diag *out, double *part, long long part_slots);
+// 4-double partial slots tbec_diagnose needs for the handle's grid, tile and segment
+long long diagnose_partial_slots(tbec_handle *h);
+// preparing the physics of a step at time t would allocate a device table
+bool physics_would_allocate(const tbec_handle *h, double t);
 int tbec_reserve_partials(tbec_handle *h, int nb);
 bool handle_rot(const tbec_handle *h);
 int set_error(int code, const std::string &msg);

@@ -578,6 +578,13 @@ int prepare_physics(tbec_handle *h, double t) {
   return 0;
 }
 
+bool physics_would_allocate(const tbec_handle *h, double t) {
+  const StepPhys sp = step_phys(h, t);
+  if (sp.mode != PH_GEN) return false;
+  if (!h->vtab) return true;
+  return !(h->vtab_valid && same_key(h->vkey, sp.key)) && !h->coords;
+}
+
 void free_physics(tbec_handle *h) {
   if (h->vtab) (void)hipFree(h->vtab);
   if (h->trap) (void)hipFree(h->trap);

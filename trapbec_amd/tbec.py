@@ -92,7 +92,7 @@ class TbecError(RuntimeError):
 
 EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi",
            "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
-           "tbec_diagnose", "tbec_imag_solve", "tbec_set_norm", "tbec_sync", "tbec_stream",
+           "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve", "tbec_set_norm", "tbec_sync", "tbec_stream",
            "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version",
            "tbec_batch_create", "tbec_batch_destroy", "tbec_batch_upload_psi",
            "tbec_batch_download_psi", "tbec_batch_imag_solve", "tbec_batch_set_tuning",
@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
         getattr(L, f).argtypes = [H, vp]
     L.tbec_step.argtypes = [H, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     L.tbec_diagnose.argtypes = [H, C.POINTER(TbecDiag)]
+    L.tbec_diagnose_noalloc.argtypes = [H, C.POINTER(TbecDiag)]
     L.tbec_imag_solve.argtypes = [H, C.c_int64, C.c_double, C.c_double, C.c_int,
                                   C.POINTER(TbecSolveResult)]
     L.tbec_set_norm.argtypes = [H, C.c_double]
@@ -257,6 +258,12 @@ class Condensate:
     def diagnose(self) -> dict:
         d = TbecDiag()
         _check(load().tbec_diagnose(self._h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in TbecDiag._fields_}
+
+    def diagnose_noalloc(self) -> dict:
+        """tbec_diagnose_noalloc: the same diagnostics with no device allocation."""
+        d = TbecDiag()
+        _check(load().tbec_diagnose_noalloc(self._h, C.byref(d)))
         return {k: getattr(d, k) for k, _ in TbecDiag._fields_}
 
     def imag_solve(self, max_steps: int, tol: float, norm: float = 1.0,
