@@ -1,0 +1,46 @@
+"""The autotuner on the GPU: every (tile, segment, XCD order) candidate is timed and the
+recorded choice is the fastest of its own sweep; the choice changes speed only (the
+tuned and the default launch give the same psi); the batch tuner likewise."""
+import numpy as np
+import pytest
+
+from trapbec_amd import autotune, initial, tbec
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tune_records_the_fastest_candidate(tmp_path):
+    shape, prec = (40, 36, 48), 32
+    table = str(tmp_path / "tuning.json")
+    print(f"\n[case] autotune {shape} fp{prec}", flush=True)
+    e = autotune.tune(shape, prec, nstages=6, table=table)
+    assert len(e["all"]) == 2 * tbec.NUM_TILES * len(autotune.SEGS)
+    best = min(a[3] for a in e["all"])
+    assert round(e["us_per_stage"], 2) == best
+    assert [e["tile"], e["seg"], e["remap"]] == e["all"][0][:3]
+    assert autotune.lookup(shape, prec, table=table) == (e["tile"], e["seg"], e["remap"])
+    # the recorded choice changes the launch shape, not the result
+    pr = initial.Problem(shape, (8.0, 7.0, 9.0), (1.0, 1.1, 0.9), 50.0, 0.4, 1e-3)
+    psi0 = initial.tf_profile(shape, pr.box, pr.omega, pr.g, norm=1.0)[0] * \
+        initial.vortex_line_factor(shape, pr.box, 0.3, 0.1, 1, 0.4)
+    out = []
+    for tile, seg, remap in ((e["tile"], e["seg"], e["remap"]), (2, 0, 1)):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.set_tuning(tile, seg)
+            c.set_xcd_remap(remap)
+            c.upload(psi0)
+            c.step(40)
+            out.append(c.download())
+    err = float(np.abs(out[0] - out[1]).max() / np.abs(out[1]).max())
+    print(f"tuned {e['tile'], e['seg'], e['remap']} vs default: rel max {err:.2e}", flush=True)
+    assert err < 2e-6
+
+
+def test_tune_batch_picks_the_fastest_candidate():
+    probs = [p for p, _ in initial.config_c4(seed=3, n_inst=3, dx=0.4, dtau=4e-3,
+                                              grids=((24, 24, 24), (32, 20, 28)))]
+    print(f"\n[case] tune_batch {[p.shape for p in probs]}", flush=True)
+    e = autotune.tune_batch(probs, prec=32, steps=8)
+    assert len(e["all"]) == tbec.NUM_TILES * len(autotune.SEGS)
+    assert [e["tile"], e["seg"]] == e["all"][0][:2]
+    assert round(e["ms_per_step"], 4) == min(a[2] for a in e["all"])

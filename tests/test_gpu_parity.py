@@ -302,8 +302,42 @@ def test_fp32_compensated_vs_fp64():
           f"reference-order {e_naive:.3e}")
     assert e_comp < 1e-5 and e_cent < 1e-5
     # at h = 1/4 the FP32 storage rounding of psi dominates; the compensated sums must
-    # not be worse than the reference order (they win where h is small, see below)
+    # not be worse than the reference order (they win where h is small:
+    # test_fp32_rhs_compensated_sums_at_fine_h)
     assert e_comp <= 1.01 * e_naive and e_cent <= 1.01 * e_naive
+
+
+def test_fp32_rhs_compensated_sums_at_fine_h():
+    """Where the compensated sums earn their cost: one FP32 get_rhs at h = 1/64 (192^3 on
+    [-1.5,1.5)^3, a Gaussian of width 1/4 times a plane wave, no potential, no
+    nonlinearity, so rhs = the Laplacian) in each summation mode, against the FP64
+    oracle's rhs of the same FP32-rounded psi.  The reference order's terms are
+    ~490 f/(180 h^2), so its rounding is ~1e4 eps |f| against a Laplacian of ~16 |f|
+    (CPU emulation: 2.9e-5 relative L2; MI355X, with fused multiply-adds: 1.4e-5); the
+    TwoSum and centred forms round relative to h^2 f'' (MI355X: 4.3e-8 and 5.3e-8).
+    Required: reference order above 5e-6, compensated forms below 3e-7 and at least 50x
+    better."""
+    n, L, w = 192, 3.0, 16.0
+    announce(f"fp32 rhs per summation mode at h = 1/64 ({n}^3)")
+    X, Y, Z = initial.mesh((n, n, n), (L, L, L))
+    psi = (np.exp(-w * (X ** 2 + Y ** 2 + Z ** 2) / 2) *
+           np.exp(1j * (0.9 * X - 0.4 * Y + 0.3 * Z))).astype(np.complex64)
+    del X, Y, Z
+    p = oracle.params((n, n, n), (L, L, L), (0.0, 0.0, 0.0), c0=0.0, gnl=0.0)
+    ref = oracle.Stepper(p, psi.astype(np.complex128)).get_rhs()
+    err = {}
+    for sm in (tbec.SUM_REFERENCE, tbec.SUM_COMPENSATED, tbec.SUM_CENTRED):
+        with tbec.Condensate((n, n, n), (L, L, L), (0.0, 0.0, 0.0), c0=0.0, gnl=0.0,
+                             prec=32, sum_mode=sm) as c:
+            c.upload(psi)
+            got = c.rhs().astype(np.complex128)
+        err[sm] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"fp32 rhs rel L2: reference order {err[0]:.3e}  twosum {err[1]:.3e}  "
+          f"centred {err[2]:.3e}", flush=True)
+    assert err[tbec.SUM_REFERENCE] > 5e-6
+    for sm in (tbec.SUM_COMPENSATED, tbec.SUM_CENTRED):
+        assert err[sm] < 3e-7
+        assert err[sm] * 50 <= err[tbec.SUM_REFERENCE]
 
 
 def test_fp32_vs_fp64_512_1000_steps():

@@ -31,20 +31,25 @@ def _load():
     return {}
 
 
-def lookup(shape, prec, sum_mode=None, w_stored=False, rot=False):
+def lookup(shape, prec, sum_mode=None, w_stored=False, rot=False, table=None):
     """(tile, seg, remap) recorded for this grid, precision, summation, w scheme and
-    rotating frame (real time)."""
-    t = _load().get(_key(shape, prec, sum_mode, w_stored, rot))
+    rotating frame (real time), in `table` (default trapbec_amd/tuning.json)."""
+    if table is None:
+        recs = _load()
+    else:
+        recs = json.load(open(table)) if os.path.exists(table) else {}
+    t = recs.get(_key(shape, prec, sum_mode, w_stored, rot))
     if t:
         return int(t["tile"]), int(t["seg"]), int(t.get("remap", 1))
     return DEFAULT[prec] + (1,)
 
 
-def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=False,
+def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=True,
          table=None, w_stored=False, problem=None, rot=False):
     """Time every (tile, seg) for this grid, precision, summation and w scheme (stages
     1-3 in turn); return the entry.  With `problem` (an initial.Problem) the stages of
-    that problem are timed (rotation, imaginary time) and nothing is saved."""
+    that problem are timed (rotation, imaginary time) and nothing is saved.  With
+    `verbose` (the default) one line per candidate goes to stdout as it is timed."""
     import numpy as np
     from trapbec_amd import tbec
     if sum_mode is None:
@@ -85,10 +90,11 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=F
     return entry
 
 
-def tune_batch(problems, prec=32, sum_mode=None, steps=20, verbose=False):
+def tune_batch(problems, prec=32, sum_mode=None, steps=20, verbose=True):
     """Time every (tile, seg) for a batched ensemble: `steps` imaginary-time steps of all
     instances (one captured graph, constant states, no convergence stop).  Returns
-    {"tile", "seg", "ms_per_step", "all"}; the caller applies it with Batch.set_tuning."""
+    {"tile", "seg", "ms_per_step", "all"}; the caller applies it with Batch.set_tuning.
+    With `verbose` (the default) one line per candidate goes to stdout as it is timed."""
     import time
     import numpy as np
     from trapbec_amd import tbec
