@@ -136,6 +136,15 @@ int tbec_diagnose_noalloc(tbec_handle *h, tbec_diag *out);
 int tbec_imag_solve(tbec_handle *h, int64_t max_steps, double tol, double norm,
                     int check_every, tbec_solve_result *out);
 
+/* Residual gate of the imaginary-time stop rule: with rmax > 0 a step counts toward
+ * convergence (tbec_imag_solve, tbec_batch_imag_solve) only if its input's GP residual
+ * ||(H - mu) psi|| / ||psi|| is at or below rmax as well, so a state that drifts slowly
+ * far from any stationary state (the vortex-free saddle of a rotating trap) cannot meet
+ * the rule.  0 (the default) disables the gate.  Takes effect at the next solve.  No
+ * counterpart in the reference, whose imaginary time has no stop rule
+ * (src/run.f90:19-35). */
+int tbec_set_residual_gate(tbec_handle *h, double rmax);
+
 /* Output-time diagnostics of the current psi, as the reference's var_print writes
  * them when vel_print / qpress_print are set (src/run.f90:64-66,
  * src/output.f90:80-91).  Both fill 3*nx*ny*nz reals of the working precision,
@@ -322,6 +331,8 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
                           int check_every, tbec_solve_result *out);
 /* Tile and segment for every instance (seg = 0: automatic over the whole ensemble). */
 int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg);
+/* tbec_set_residual_gate for every instance of the batch (next solve). */
+int tbec_batch_set_residual_gate(tbec_batch *b, double rmax);
 void *tbec_batch_stream(tbec_batch *b);
 
 /* ---- learned warm start ---------------------------------------------------------

@@ -225,11 +225,12 @@ class Stepper:
         return out
 
     def imag_rk3(self, nsteps: int, norm_target: float, tol: float = 0.0,
-                 hits: int = 8) -> list:
+                 hits: int = 8, rgate: float = 0.0) -> list:
         """Renormalised imaginary-time RK3 (dt must be (0, -dtau)).  Stops early once the
         relative energy change between consecutive steps has stayed below tol for `hits`
-        steps in a row.  Returns the per-step diagnostics [N, E, mu, residual] of the
-        state entering each step."""
+        steps in a row; with rgate > 0 a step counts only if the GP residual of its input
+        is at or below rgate as well.  Returns the per-step diagnostics
+        [N, E, mu, residual] of the state entering each step."""
         assert self.prec == 64
         if not hasattr(self, "c0_eff"):
             self.c0_eff = np.array([self.p.c0])
@@ -242,7 +243,9 @@ class Stepper:
                                         _ptr(diag))
             hist.append(diag.copy())
             if tol > 0 and len(hist) > 1:
-                run = run + 1 if abs(hist[-1][1] - hist[-2][1]) < tol * abs(hist[-1][1]) else 0
+                ok = abs(hist[-1][1] - hist[-2][1]) < tol * abs(hist[-1][1]) and \
+                    (rgate <= 0 or hist[-1][3] <= rgate)
+                run = run + 1 if ok else 0
                 if run >= hits:
                     break
         return hist

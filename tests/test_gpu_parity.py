@@ -240,6 +240,39 @@ def test_imag_solve_stop_step_vs_oracle():
     assert relmax(got, s2.interior) < 1e-11
 
 
+@pytest.mark.parametrize("path", ["single", "batch"])
+def test_imag_solve_residual_gate_vs_oracle(path):
+    """With a GP-residual gate of 3e-4 a step counts toward the stop rule only if its
+    input's residual is at or below the gate: the solve stops where the oracle's gated
+    rule fires (830 steps here, against 458 ungated), single handle and batch alike."""
+    shape, box, om, g = (24, 24, 24), (12.0, 12.0, 12.0), (1.0, 1.1, 0.9), 20.0
+    announce(f"imag solve residual gate {shape} {path}")
+    pr = initial.Problem(shape, box, om, g, 0.3, -5e-3j)
+    kp = pr.kernel_params()
+    psi0 = initial.tf_profile(shape, box, om, g, norm=1.0)[0] * \
+        initial.phase_noise(shape, 5, 0.1)
+    s = oracle.Stepper(oracle.params(shape, box, kp["harm_w"], omega_z=kp["omega_r"],
+                                     c0=kp["c0"], gnl=kp["gnl"], dt=kp["dt"]), psi0)
+    hist = s.imag_rk3(5000, 1.0, tol=1e-7, rgate=3e-4)
+    m = len(hist) - 1
+    assert 600 < m < 4999 and hist[-1][3] <= 3e-4
+    if path == "single":
+        with tbec.Condensate.from_problem(pr, prec=64) as c:
+            c.set_residual_gate(3e-4)
+            c.upload(psi0)
+            r = c.imag_solve(5000, 1e-7, 1.0, 50)
+    else:
+        with tbec.Batch([pr, pr], prec=64) as b:
+            b.set_residual_gate(3e-4)
+            b.upload(0, psi0)
+            b.upload(1, psi0)
+            r = b.imag_solve(5000, 1e-7, 1.0, 50)[1]
+    assert r["converged"] == 1
+    assert r["steps"] == m
+    assert r["energy"] == pytest.approx(hist[-1][1], rel=1e-12)
+    assert r["residual"] <= 3e-4
+
+
 def test_imag_solve_converges_harmonic():
     """g = 0 isotropic trap from a displaced Gaussian: converges to E = 3/2 (FP64)."""
     shape, box = (32, 32, 32), (12.0, 12.0, 12.0)

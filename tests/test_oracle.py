@@ -660,3 +660,24 @@ def test_psi_info_brute_force():
     # (a per-rank count) and sum|psi| in place of sum|psi|^2
     assert abs(info[2] - 4 * info[2]) > 1e-3 * info[2]
     assert abs(info[3] - math.fsum(a.ravel())) > 1e-3 * info[3]
+
+
+def test_imag_rk3_residual_gate():
+    """The gated stop rule: with rgate the run continues past the ungated stop until the
+    residual of the step's input is at or below the gate, and ends at a lower energy."""
+    shape, box, om, g = (16, 16, 16), (10.0, 10.0, 10.0), (1.0, 1.1, 0.9), 20.0
+    pr = initial.Problem(shape, box, om, g, 0.3, -5e-3j)
+    kp = pr.kernel_params()
+    psi0 = initial.tf_profile(shape, box, om, g, norm=1.0)[0] * \
+        initial.phase_noise(shape, 5, 0.1)
+    mk = lambda: oracle.Stepper(oracle.params(shape, box, kp["harm_w"],  # noqa: E731
+                                              omega_z=kp["omega_r"], c0=kp["c0"],
+                                              gnl=kp["gnl"], dt=kp["dt"]), psi0)
+    h0 = mk().imag_rk3(20000, 1.0, tol=1e-7)
+    gate = h0[-1][3] / 4
+    h1 = mk().imag_rk3(20000, 1.0, tol=1e-7, rgate=gate)
+    assert len(h1) > len(h0) and len(h1) < 20000
+    assert h1[-1][3] <= gate and h1[-1][1] < h0[-1][1]
+    # the last 8 steps all met both conditions
+    for k in range(1, 9):
+        assert abs(h1[-k][1] - h1[-k - 1][1]) < 1e-7 * abs(h1[-k][1]) and h1[-k][3] <= gate

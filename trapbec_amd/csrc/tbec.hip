@@ -559,6 +559,14 @@ int tbec_destroy(tbec_handle *h) {
   return 0;
 }
 
+int tbec_set_residual_gate(tbec_handle *h, double rmax) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (!(rmax >= 0.0) || !std::isfinite(rmax))
+    return fail(TBEC_ERR_ARG, "residual gate must be finite and >= 0");
+  h->rgate = rmax;
+  return 0;
+}
+
 int tbec_set_norm(tbec_handle *h, double norm) {
   if (!h || !(norm > 0)) return fail(TBEC_ERR_ARG, "norm must be positive");
   h->norm_target = norm;
@@ -795,6 +803,7 @@ int tbec_reset_state(tbec_handle *h) {
   s[ST_DV] = h->hx * h->hy * h->hz;
   s[ST_NORMT] = h->norm_target;
   s[ST_TOL] = h->tol;
+  s[ST_RGATE] = h->rgate;
   s[ST_F32] = h->p.prec == TBEC_F32 ? 1.0 : 0.0;
   HIPCHK(hipMemcpyAsync(h->dstate, s, sizeof(double) * ST_COUNT, hipMemcpyHostToDevice,
                         h->stream));

@@ -37,6 +37,7 @@ struct tbec_batch {
   std::vector<int> cur;              // per-instance current buffer after a solve
   int parity = 0;
   int total_items = 0;
+  double rgate = 0.0;                // GP-residual gate of the stop rule (0: none)
   BatchInst *d_inst = nullptr;
   int *d_item_inst = nullptr;
   int *d_item_begin = nullptr;
@@ -261,6 +262,7 @@ int reset_states(tbec_batch *b, double norm, double tol) {
     s[ST_DV] = b->L[i].hx * b->L[i].hy * b->L[i].hz;
     s[ST_NORMT] = norm;
     s[ST_TOL] = tol;
+    s[ST_RGATE] = b->rgate;
     s[ST_F32] = b->prec == TBEC_F32 ? 1.0 : 0.0;
     for (int k = 0; k < 4; ++k) b->h_flags[4 * i + k] = 0;
   }
@@ -370,6 +372,14 @@ int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg) {
   b->tile = tile;
   b->seg = seg > 0 ? seg : batch_auto_seg(b, tile);
   return build_table(b);
+}
+
+int tbec_batch_set_residual_gate(tbec_batch *b, double rmax) {
+  if (!b) return set_error(TBEC_ERR_ARG, "null batch");
+  if (!(rmax >= 0.0) || !std::isfinite(rmax))
+    return set_error(TBEC_ERR_ARG, "residual gate must be finite and >= 0");
+  b->rgate = rmax;
+  return 0;
 }
 
 void *tbec_batch_stream(tbec_batch *b) { return b ? (void *)b->stream : nullptr; }

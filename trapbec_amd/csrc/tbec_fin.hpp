@@ -91,7 +91,12 @@ __device__ inline void fin_energy(const double s[4], double *st, int *flag, int 
   // above (which adds c0*N/2 back) uses exactly the c0 the rhs was computed with
   const double c0n = st[ST_C0BASE] + 2.0 * mu;
   st[ST_C0NEXT] = st[ST_F32] != 0.0 ? (double)(float)c0n : c0n;
-  st[ST_HITS] = rel < st[ST_TOL] ? st[ST_HITS] + 1.0 : 0.0;
+  // a step counts toward convergence when the energy change is below the tolerance and,
+  // with a residual gate, the GP residual of the step's input is at or below the gate:
+  // a slowly drifting state far from stationary (a vortex-free saddle of a rotating
+  // trap) then cannot meet the rule
+  const bool res_ok = st[ST_RGATE] <= 0.0 || st[ST_RES] <= st[ST_RGATE];
+  st[ST_HITS] = (rel < st[ST_TOL] && res_ok) ? st[ST_HITS] + 1.0 : 0.0;
   if (st[ST_HITS] >= (double)kConvergeHits) {
     flag[0] = 1;
     flag[1] = (int)st[ST_STEP];

@@ -28,6 +28,7 @@ enum StateSlot {
   ST_HITS,        // consecutive steps below the tolerance
   ST_DC0,         // diagnostics: 2*mu, the c0 of the residual pass
   ST_F32,         // 1 when the kernels compute in FP32 (c0 values are stored rounded)
+  ST_RGATE,       // a step counts toward convergence only with GP residual <= this (0: any)
   ST_COUNT
 };
 
@@ -98,6 +99,7 @@ struct tbec_handle {
   int *dflag = nullptr, *hflag = nullptr;
   double norm_target = 1.0;
   double tol = 0.0;
+  double rgate = 0.0;                   // GP-residual gate of the stop rule (0: none)
   bool scale_pending = false;
   hipStream_t stream = nullptr;
   // captured step graphs: (mode, steps, parity) -> executable graph
