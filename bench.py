@@ -68,6 +68,19 @@ def parse():
     ap.add_argument("--imag-probe", action="store_true",
                     help="imaginary-time solve steps of the rotating 256^3 ground-state "
                          "problem (FP32 and FP64), for kernel traces")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the ground-state leg (network start vs TF start)")
+    ap.add_argument("--warm-fine", type=int, default=384,
+                    help="grid of the ground-state leg (C5: 384^3 on [-12,12)^3)")
+    ap.add_argument("--warm-steps", type=int, default=20000,
+                    help="imaginary-time steps of the network start (the TF start may "
+                         "take up to --warm-cap times as many)")
+    ap.add_argument("--warm-cap", type=float, default=5.0)
+    ap.add_argument("--warm-samples", type=int, default=2000)
+    ap.add_argument("--mlp-probe", action="store_true",
+                    help="for kernel traces of the warm-start network (5 -> 4 x 512 -> "
+                         "98,304): forward and forward+backward at batch 32 and 256, and "
+                         "a few training steps at batch 64")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: 3 FP32 stages at n^3 plus a plain "
                          "device copy of one psi-sized buffer (the counter calibration)")
@@ -183,45 +196,117 @@ def stream_ceiling(n, prec, reps=20):
             "value": round(n ** 3 / (step_ms * 1e-3) / 1e9, 4)}
 
 
+def host_cores():
+    """The host cores this job can use, and how that was read: the logical CPUs it may
+    run on (sched_getaffinity), the cgroup CPU quota (cpu.max), and the physical cores
+    among its CPUs (lscpu -p).  `usable` is the smallest of these."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    info["affinity"] = len(cpus)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_quota"] = quota
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True,
+                             text=True, timeout=20).stdout
+        cores = set()
+        for ln in out.splitlines():
+            if ln.startswith("#") or not ln.strip():
+                continue
+            cpu, core, sock = (int(v) for v in ln.split(",")[:3])
+            if cpu in cpus:
+                cores.add((sock, core))
+        phys = len(cores) or None
+        model = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in model.splitlines():
+            if ln.startswith("Model name:"):
+                info["model"] = ln.split(":", 1)[1].strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    info["physical_cores"] = phys
+    usable = min(v for v in (info["affinity"], phys,
+                             int(quota) if quota else None) if v)
+    info["usable"] = max(1, usable)
+    return info
+
+
+def mpi_ranks(n, usable):
+    """The rank count BASELINE.md plans: the largest power of two <= the usable cores
+    whose 2-D decomposition (MPI_DIMS_CREATE) divides the n x n cross-section
+    (src/mpi_var.f90:38-41 aborts otherwise)."""
+    p = 1
+    while 2 * p <= usable and n % (2 * p) == 0:
+        p *= 2
+    return p
+
+
+def time_reference(n, prec, ranks, mpiexec):
+    """Per-step wall time of the reference Fortran (oracle/_ref, built from its own
+    sources with its Makefile's -O3) at n^3: wall(3 steps) - wall(1 step), halved, so
+    start-up and initialisation cancel."""
+    times = {}
+    for steps in (1, 3):
+        d = os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f{prec}_s{steps}")
+        if not os.path.exists(os.path.join(d, "run.x")):
+            raise FileNotFoundError(os.path.join(d, "run.x"))
+        shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
+        os.makedirs(os.path.join(d, "data"))
+        for r in range(ranks):
+            os.makedirs(os.path.join(d, "data", f"proc{r:03d}"), exist_ok=True)
+        t0 = time.perf_counter()
+        res = subprocess.run([mpiexec, "-n", str(ranks), "./run.x"], cwd=d,
+                             capture_output=True, text=True, errors="replace", timeout=900)
+        times[steps] = time.perf_counter() - t0
+        if res.returncode != 0:
+            raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+        shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
+        print(f"[bench] reference FP{prec} {steps} step(s) on {ranks} ranks: "
+              f"{times[steps]:.2f} s", file=sys.stderr, flush=True)
+    return (times[3] - times[1]) / 2.0
+
+
 def cpu_baseline(n):
-    """The reference Fortran (oracle/_ref, built from its own sources) on this host's
-    cores, 512^3 FP32: wall time of a 3-step run minus a 1-step run of the same build
-    family, i.e. per-step time without start-up.  Falls back to the oracle's C
-    restatement (OpenMP) if the reference build or MPI is absent."""
-    import numpy as np  # noqa: F401
-    cores = min(16, os.cpu_count() or 1)
-    while cores > 1 and ((n % cores) and (n % max(1, cores // 2))):
-        cores //= 2
+    """The reference Fortran on this host's usable cores, FP32 (the `value`) and FP64
+    (under "fp64"), the 512^3 workload's grid, trap and dt from the reference's own
+    `constant` initial state (the per-step cost does not depend on the data).  Falls
+    back to the oracle's C restatement (OpenMP) if the reference build or MPI is
+    absent."""
+    host = host_cores()
     mpiexec = "/opt/conda/bin/mpiexec"
-    d1 = os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f32_s1")
-    d3 = os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f32_s3")
-    if os.path.exists(os.path.join(d1, "run.x")) and os.path.exists(
-            os.path.join(d3, "run.x")) and shutil.which(mpiexec):
-        times = {}
-        for d, s in ((d1, 1), (d3, 3)):
-            shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
-            os.makedirs(os.path.join(d, "data"))
-            for r in range(cores):
-                os.makedirs(os.path.join(d, "data", f"proc{r:03d}"), exist_ok=True)
-            t0 = time.perf_counter()
-            res = subprocess.run([mpiexec, "-n", str(cores), "./run.x"], cwd=d,
-                                 capture_output=True, text=True, errors="replace",
-                                 timeout=900)
-            times[s] = time.perf_counter() - t0
-            if res.returncode != 0:
-                raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
-            shutil.rmtree(os.path.join(d, "data"), ignore_errors=True)
-        per_step = (times[3] - times[1]) / 2.0
-        return {"value": n ** 3 / per_step / 1e9, "unit": "Gpt-steps/s", "cores": cores,
-                "kind": "reference",
-                "sample": f"reference Fortran FP32 {n}^3, mpiexec -n {cores}, wall(3 steps) "
-                          f"- wall(1 step) = {2 * per_step:.2f} s for 2 steps",
-                "s_per_step": per_step}
+    ranks = mpi_ranks(n, host["usable"])
+    have = all(os.path.exists(os.path.join(ROOT, "oracle", "_ref", f"bench{n}_f{p}_s{s}",
+                                           "run.x")) for p in (32, 64) for s in (1, 3))
+    if have and shutil.which(mpiexec):
+        per = {p: time_reference(n, p, ranks, mpiexec) for p in (32, 64)}
+        dims = f"{ranks} ranks"
+        out = {"value": n ** 3 / per[32] / 1e9, "unit": "Gpt-steps/s", "cores": ranks,
+               "kind": "reference",
+               "sample": f"reference Fortran FP32 {n}^3 (amdflang -O3, MPICH), mpiexec -n "
+                         f"{ranks}: wall(3 steps) - wall(1 step) = {2 * per[32]:.2f} s for "
+                         f"2 steps",
+               "s_per_step": per[32],
+               "fp64": {"value": n ** 3 / per[64] / 1e9, "unit": "Gpt-steps/s",
+                        "cores": ranks, "s_per_step": per[64],
+                        "sample": f"reference Fortran FP64 (-fdefault-real-8) {n}^3, "
+                                  f"{dims}, same method"},
+               "host": host}
+        return out
     # fallback: the C restatement with OpenMP threads, 1 step at n^3
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import numpy as np
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cores = host["usable"]
+    os.environ["OMP_NUM_THREADS"] = str(cores)
     p = oracle.params((n, n, n), 32.0, (1024.0,) * 3, c0=0.0, gnl=1000.0, dt=5e-4)
     s = oracle.Stepper(p, np.full((n, n, n), 0.1 + 0.05j, np.complex64), prec=32)
     t0 = time.perf_counter()
@@ -229,7 +314,77 @@ def cpu_baseline(n):
     dt = time.perf_counter() - t0
     return {"value": n ** 3 / dt / 1e9, "unit": "Gpt-steps/s", "cores": cores,
             "kind": "port", "sample": f"oracle C restatement FP32 {n}^3, 1 step, "
-                                      f"{cores} OpenMP threads", "s_per_step": dt}
+                                      f"{cores} OpenMP threads", "s_per_step": dt,
+            "host": host}
+
+
+WARM_TEST = (1.0, 0.8, 3000.0, 0.5, 0.0)   # w_perp, w_z, g, Omega, log N (held out)
+
+
+def warm_start_leg(args):
+    """The ground-state half of the metric (C5): the warm-start network (5 -> 4 x 512 GELU
+    -> 32^3 x 3, BF16 MFMA) trained on the GPU on ground states from the batched solver,
+    then, at a held-out rotating problem on args.warm_fine^3, the network's start against
+    the plain TF start at equal final energy and GP residual: the network start runs
+    args.warm_steps FP32 imaginary-time steps (dtau = 8e-4), and the TF start runs until
+    it reaches that energy and residual, or args.warm_cap times the steps."""
+    from trapbec_amd import warmstart as ws
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    m, scale, info = ws.train_network(samples=args.warm_samples, log=log)
+    try:
+        coarse, infer_s = ws.predict(m, scale, WARM_TEST)
+    finally:
+        m.close()
+    dtau = 8e-4
+    pr = ws.fine_problem(WARM_TEST, args.warm_fine, dtau)
+    r = ws.equal_state_comparison(pr, coarse, args.warm_steps, args.warm_cap,
+                                  tuning=(2, 32, 1), infer_s=infer_s, log=log)
+    r.pop("records")
+    out = {"metric": "ground-state time, network (warm) start vs TF start, at equal final "
+                     "energy and GP residual",
+           "test_params": {"w_perp": WARM_TEST[0], "w_z": WARM_TEST[1], "g": WARM_TEST[2],
+                           "Omega": WARM_TEST[3], "log_N": WARM_TEST[4]},
+           "grid": [args.warm_fine] * 3, "box": [-12.0, 12.0], "dtau": dtau,
+           "dtype": "f32 (FP64 reductions)", "network": info, "inference_s": infer_s, **r}
+    sp = r.get("speedup_steps")
+    log(f"[bench] warm start: network {r['network_start']['steps']} steps "
+        f"{r['network_start']['wall_s']:.1f} s; TF "
+        f"{'reached' if r['tf_start']['reached'] else 'did not reach'} the target in "
+        f"{r['tf_start']['steps']} steps {r['tf_start']['wall_s']:.1f} s; speed-up "
+        f"{sp if sp else '>= %.2f' % r['speedup_steps_lower_bound']}")
+    return out
+
+
+def mlp_probe(args):
+    """The warm-start network's kernels at production size, for kernel traces: 5 forward
+    passes and 5 loss/gradient passes at batch 32 and at batch 256, then one epoch of
+    training at batch 64 over 256 samples (Adam and the BF16 weight refresh).  Prints host
+    wall times per call (they include the host copies of the outputs)."""
+    import numpy as np
+    from trapbec_amd import tbec
+    rng = np.random.default_rng(1)
+    n_out = 3 * 32 ** 3
+    res = {"mlp_probe": "5 -> 4 x 512 GELU -> 98304"}
+    with tbec.MLP(5, 512, 4, n_out, seed=1) as m:
+        for B in (32, 256):
+            X = rng.uniform(-1, 1, (B, 5)).astype(np.float32)
+            T = (0.3 * rng.standard_normal((B, n_out))).astype(np.float32)
+            m.forward(X)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                m.forward(X)
+            res[f"forward_b{B}_ms"] = round((time.perf_counter() - t0) * 200, 3)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                m.loss_grad(X, T, with_grads=False)
+            res[f"loss_grad_b{B}_ms"] = round((time.perf_counter() - t0) * 200, 3)
+        X = rng.uniform(-1, 1, (256, 5)).astype(np.float32)
+        T = (0.3 * rng.standard_normal((256, n_out))).astype(np.float32)
+        m.set_data(X, T)
+        t0 = time.perf_counter()
+        m.train(1, 64, 1e-3, seed=2)
+        res["train_step_b64_ms"] = round((time.perf_counter() - t0) * 1e3 / 4, 3)
+    print(json.dumps(res), flush=True)
 
 
 def pmc_probe(args):
@@ -388,6 +543,9 @@ def main():
     if args.pmc_probe:
         pmc_probe(args)
         return
+    if args.mlp_probe:
+        mlp_probe(args)
+        return
     if args.extensions:
         extensions_probe(args)
         return
@@ -435,7 +593,8 @@ def main():
     head = 32 if 32 in res else 64
     r = res[head]
     out = {
-        "metric": "Gpt-steps/s at 512^3 FP32/FP64 (% HBM peak)",
+        "metric": "Gpt-steps/s at 512\u00b3 FP32/FP64 (% HBM peak); ground-state time warm "
+                  "vs TF start",
         "value": round(r["gpt_steps_per_s"], 4),
         "unit": "Gpt-steps/s",
         "n_gpus": 1,
@@ -492,6 +651,13 @@ def main():
         with open(traffic) as fh:
             out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch_fp32")
             out["roofline"]["traffic_unit"] = "bytes per launch (FP32 headline kernel)"
+    if not args.no_warm_start:
+        print("[bench] ground-state leg (network start vs TF start)", file=sys.stderr,
+              flush=True)
+        try:
+            out["warm_start"] = warm_start_leg(args)
+        except Exception as e:  # reported, never silently replaced
+            out["warm_start"] = {"value": None, "error": str(e)[-500:]}
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         try:

@@ -110,8 +110,11 @@ def cparam_text(p: dict) -> str:
     ])
 
 
-def build(name: str, prec: int = 64, contract: str = "off", **params) -> str:
-    """Compile one reference variant into oracle/_ref/<name>/run.x; return its directory."""
+def build(name: str, prec: int = 64, contract: str | None = "off", **params) -> str:
+    """Compile one reference variant into oracle/_ref/<name>/run.x; return its directory.
+    contract: the -ffp-contract mode (fixtures use "off", so the C restatement can match
+    bit for bit); None leaves the compiler's default, i.e. the reference Makefile's plain
+    -O3 (src/Makefile:2), which the CPU timing builds use."""
     if not os.path.isdir(REF_SRC):
         raise FileNotFoundError(f"reference sources not found at {REF_SRC}")
     p = dict(DEFAULTS)
@@ -126,7 +129,9 @@ def build(name: str, prec: int = 64, contract: str = "off", **params) -> str:
         os.symlink(os.path.join(REF_SRC, m + ".f90"), os.path.join(d, m + ".f90"))
     with open(os.path.join(d, "cparam.local"), "w") as f:
         f.write(cparam_text(p))
-    flags = ["-O3", f"-ffp-contract={contract}", f"-I{MPI_INC}"]
+    flags = ["-O3", f"-I{MPI_INC}"]
+    if contract is not None:
+        flags.insert(1, f"-ffp-contract={contract}")
     if prec == 64:
         flags += ["-fdefault-real-8", "-cpp", "-DMPI_REAL=MPI_DOUBLE_PRECISION",
                   "-DMPI_COMPLEX=MPI_DOUBLE_COMPLEX"]

@@ -25,10 +25,11 @@ def test_bench_refuses_more_than_one_gpu():
 
 @pytest.mark.gpu
 def test_bench_json_line_at_64():
-    print("\n[case] bench.py --n 64 --steps 20 --warmup 4 --fp64-steps 10 --no-cpu",
-          flush=True)
+    print("\n[case] bench.py --n 64 --steps 20 --warmup 4 --fp64-steps 10 --no-cpu, "
+          "ground-state leg at 64^3 with 64 training states", flush=True)
     r = run_bench("--n", "64", "--steps", "20", "--warmup", "4", "--fp64-steps", "10",
-                  "--no-cpu")
+                  "--no-cpu", "--warm-fine", "64", "--warm-samples", "64",
+                  "--warm-steps", "2000")
     assert r.returncode == 0, r.stderr[-3000:]
     lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
     assert len(lines) == 1
@@ -55,6 +56,12 @@ def test_bench_json_line_at_64():
                                           abs=2e-4)
     assert sc["frac"] == pytest.approx(out["value"] / sc["value"], rel=1e-3)
     assert out["fp64"]["value"] > 0
+    ws = out["warm_start"]
+    assert "error" not in ws, ws
+    assert ws["grid"] == [64, 64, 64] and ws["network"]["samples"] == 64
+    assert ws["network_start"]["steps"] == 2000
+    assert ws["tf_cap_steps"] == 10000 and ws["tf_start"]["steps"] <= 10000
+    assert ("speedup_steps" in ws) != ("speedup_steps_lower_bound" in ws)
 
 
 @pytest.mark.gpu
@@ -67,7 +74,7 @@ def test_bench_short_run_agrees_with_long_run():
         print(f"\n[case] bench.py --n 512 --only f32 --headline-only --steps {steps} "
               f"--warmup {warm}", flush=True)
         r = run_bench("--n", "512", "--only", "f32", "--headline-only", "--no-cpu",
-                      "--steps", str(steps), "--warmup", str(warm))
+                      "--no-warm-start", "--steps", str(steps), "--warmup", str(warm))
         assert r.returncode == 0, r.stderr[-3000:]
         ms[steps] = json.loads(r.stdout.strip().splitlines()[-1])["ms_per_step"]
         print(f"[case] {steps} steps: {ms[steps]} ms/step", flush=True)

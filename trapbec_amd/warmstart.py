@@ -182,6 +182,119 @@ def history(pr, max_steps, chunk=1000, coarse=None, start=None, prec=32, tuning=
     return {"steps": steps, "wall_s": wall, "history": out}
 
 
+def _start_handle(pr, coarse=None, start=None, prec=32, tuning=None):
+    c = tbec.Condensate.from_problem(pr, prec=prec)
+    if tuning is not None:
+        c.set_tuning(tuning[0], tuning[1])
+        c.set_xcd_remap(tuning[2])
+    if coarse is not None:
+        tbec.upload_coarse(c, coarse)
+    else:
+        c.upload(start)
+    return c
+
+
+def run_to_state(c, max_steps, chunk, e_target=None, r_target=None, log=print, tag=""):
+    """Renormalised imaginary-time steps on handle c in chunks of `chunk`, with the FP64
+    diagnostics (energy, GP residual) after every chunk, until both are at or below the
+    targets (when given) or max_steps.  Returns steps, wall time (first step to last
+    diagnostics, device-synchronised) and the (steps, E, residual) record."""
+    c.sync()
+    t0 = last = time.time()
+    d = c.diagnose()
+    rec = [(0, d["energy"], d["residual"])]
+    steps = 0
+
+    def reached(dd):
+        return e_target is not None and dd["energy"] <= e_target and dd["residual"] <= r_target
+
+    while steps < max_steps and not reached(d):
+        n = min(chunk, max_steps - steps)
+        c.step(n, tbec.IMAG_RK3)
+        steps += n
+        d = c.diagnose()
+        rec.append((steps, d["energy"], d["residual"]))
+        if time.time() - last > 20:
+            log(f"[warmstart] {tag} {steps} steps: E {d['energy']:.10f} residual "
+                f"{d['residual']:.3e}")
+            last = time.time()
+    return {"steps": steps, "wall_s": time.time() - t0, "reached": reached(d),
+            "energy": d["energy"], "residual": d["residual"], "record": rec}
+
+
+def equal_state_comparison(pr, coarse, warm_steps, cap_factor=5.0, chunk=1000,
+                           tuning=None, infer_s=0.0, log=print):
+    """The network start runs `warm_steps` steps and fixes the target: its energy and GP
+    residual.  The TF start (the same seeded TF profile and phase noise the training
+    states start from) then runs until it reaches both targets or cap_factor times the
+    network start's steps.  Both runs use the same step loop and diagnostics cadence, so
+    steps and wall times compare directly.  Returns the two runs and the speed-up (a
+    lower bound when the TF start does not reach the target within the cap)."""
+    t0 = time.time()
+    c = _start_handle(pr, coarse=coarse, tuning=tuning)
+    up_s = time.time() - t0
+    warm = run_to_state(c, warm_steps, chunk, log=log, tag="network start")
+    c.close()
+    warm["wall_s"] += up_s + infer_s
+    cap = int(cap_factor * warm["steps"])
+    t0 = time.time()
+    c = _start_handle(pr, start=tf_start(pr, 1.0, seed=0), tuning=tuning)
+    up_s = time.time() - t0
+    tf = run_to_state(c, cap, chunk, e_target=warm["energy"], r_target=warm["residual"],
+                      log=log, tag="TF start")
+    c.close()
+    tf["wall_s"] += up_s
+    out = {"network_start": {k: v for k, v in warm.items() if k != "record"},
+           "tf_start": {k: v for k, v in tf.items() if k != "record"},
+           "tf_cap_steps": cap,
+           "target": {"energy": warm["energy"], "residual": warm["residual"]}}
+    if tf["reached"]:
+        out["speedup_steps"] = tf["steps"] / warm["steps"]
+        out["speedup_wall"] = tf["wall_s"] / warm["wall_s"]
+    else:
+        out["speedup_steps_lower_bound"] = cap / warm["steps"]
+        out["speedup_wall_lower_bound"] = tf["wall_s"] / warm["wall_s"]
+    out["records"] = {"network_start": warm["record"], "tf_start": tf["record"]}
+    return out
+
+
+def train_network(samples=2000, train_n=64, gen_max=8000, gen_tol=1e-8, epochs=300,
+                  batch=64, lr=1e-3, seed=11, log=print):
+    """Training data from the batched solver, then the network trained on the GPU.
+    Returns (mlp handle, output scale, stats); the caller closes the handle."""
+    t0 = time.time()
+    params, X, Y, stats = generate(samples, seed=seed, n=train_n, tol=gen_tol,
+                                   max_steps=gen_max, log=log)
+    t_gen = time.time() - t0
+    scale = np.ones(Y.shape[1], np.float32)
+    scale[:COARSE ** 3] = 1.0 / Y[:, :COARSE ** 3].max()     # sqrt-density to O(1)
+    m = tbec.MLP(5, 512, 4, 3 * COARSE ** 3, seed=1)
+    n_tr = int(0.9 * samples)
+    m.set_data(X[:n_tr], Y[:n_tr] * scale)
+    t1 = time.time()
+    hist = m.train(epochs, batch, lr, seed=2)
+    t_train = time.time() - t1
+    val = m.forward(X[n_tr:]) / scale
+    info = {"samples": samples, "train_grid": train_n, "gen_s": t_gen, "train_s": t_train,
+            "gen_converged": int(sum(s["converged"] for s in stats)),
+            "epochs": epochs, "loss_first_last": [float(hist[0]), float(hist[-1])],
+            "val_mse": float(np.mean((val - Y[n_tr:]) ** 2)),
+            "val_mse_mean_predictor": float(np.mean((Y[:n_tr].mean(axis=0) - Y[n_tr:]) ** 2))}
+    log(f"[warmstart] data {t_gen:.1f} s ({info['gen_converged']}/{samples} converged), "
+        f"training {t_train:.1f} s, val MSE {info['val_mse']:.4f} (mean predictor "
+        f"{info['val_mse_mean_predictor']:.4f})")
+    return m, scale, info
+
+
+def predict(m, scale, test):
+    """Coarse field (3, 32, 32, 32) for parameters `test`, at norm 1, and the time taken."""
+    t0 = time.time()
+    coarse = (m.forward(features(np.asarray(test)[None]))[0] / scale).reshape(
+        3, COARSE, COARSE, COARSE)
+    coarse[0] /= math.sqrt(math.exp(test[4]))   # the refinement runs at norm 1
+    return coarse, time.time() - t0
+
+
 def first_reaching(hist, e_target, r_target=None):
     """First recorded step count with E <= e_target (and residual <= r_target)."""
     for st, e, r in hist:
