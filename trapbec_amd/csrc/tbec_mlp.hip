@@ -3,12 +3,19 @@
 // the trilinear upsampling that turns its coarse prediction into a starting state.
 // No reference counterpart (the reference has no learned component).
 //
-// GEMM: C[M x N] = A[M x K] * B[K x N] with arbitrary strides (so transposed operands
-// need no copies).  A 64x64 output tile per 256-thread workgroup; each wave owns 32x32
-// as 2x2 fragments of v_mfma_f32_16x16x32_bf16.  K advances 32 at a time through LDS,
-// where both operands are stored k-contiguous: As[m][k], Bs[n][k].  Operands are FP32 in
-// memory and rounded to BF16 (nearest-even) when staged; accumulation is FP32.  Long-K
-// products are split over workgroups and the partials summed in a fixed order.
+// GEMMs on v_mfma_f32_16x16x32_bf16, BF16 operands, FP32 accumulation:
+// - k_gemm_bt, for every product whose operands are both contiguous in the reduction
+//   index (forward C = H W, and dH = dZ W^T): the weight operand is a BF16 copy of the
+//   weights kept in the layout that makes it so (W^T for the forward, W itself for dH),
+//   refreshed once after every parameter update; activations are FP32 rounded to BF16
+//   (nearest-even) when staged.  A 64x128 output tile per 256-thread workgroup (4 waves of
+//   32x64 = 2x4 fragments), K in steps of 32 through double-buffered LDS (27.6 KB, so
+//   four workgroups share a CU) with 16-byte global loads, the next step's loads in
+//   flight while the current step's 8 MFMAs per wave run, one barrier per step.
+// - k_gemm, the generic strided form (dW = H^T dZ, where the reduction runs over the
+//   batch, and the 5-wide input layer): a 64x64 tile, operands rounded to BF16 when
+//   staged.
+// Long-K products are split over workgroups and the partials summed in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -133,6 +140,157 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmArgs g) {
       }
 }
 
+constexpr int TBM = 64, TBN = 128, TBK = 32, TPAD = 8;
+
+struct GemmBtArgs {
+  const float *A;           // A[m][k] = A[m*lda + k], FP32
+  long long lda;
+  const uint16_t *Bt;       // B[k][n] = Bt[n*ldb + k], BF16 bits
+  long long ldb;
+  float *C;                 // row-major M x N; EPI_PARTIAL: [split][M][N]
+  float *Z;
+  const float *bias;
+  int M, N, K;
+  int ksplit;
+  int epi;
+};
+
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+__global__ void __launch_bounds__(256) k_gemm_bt(const GemmBtArgs g) {
+  __shared__ __align__(16) uint16_t As[2][TBM][TBK + TPAD];
+  __shared__ __align__(16) uint16_t Bs[2][TBN][TBK + TPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * TBM, n0 = blockIdx.x * TBN;
+  const int kper = ((g.K + g.ksplit - 1) / g.ksplit + TBK - 1) / TBK * TBK;
+  const int kbeg = blockIdx.z * kper;
+  const int kend = min(g.K, kbeg + kper);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  f32x4 acc[2][4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // staging: A as 2 float4 per thread (row idx/8, k (idx%8)*4), B as 2 x 16 bytes per
+  // thread (row idx/4, k (idx%4)*8); elements outside [M] x [N] x [kbeg, kend) are zero
+  float4 ra[2];
+  uint4 rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = q * 256 + tid, r = idx >> 3, k = k0 + (idx & 7) * 4, m = m0 + r;
+      if (m < g.M && k + 3 < kend) {
+        ra[q] = *reinterpret_cast<const float4 *>(g.A + (long long)m * g.lda + k);
+      } else {
+        float v[4];
+        for (int e = 0; e < 4; ++e)
+          v[e] = (m < g.M && k + e < kend) ? g.A[(long long)m * g.lda + k + e] : 0.f;
+        ra[q] = make_float4(v[0], v[1], v[2], v[3]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = q * 256 + tid, r = idx >> 2, k = k0 + (idx & 3) * 8, n = n0 + r;
+      if (n < g.N && k + 7 < kend) {
+        rb[q] = *reinterpret_cast<const uint4 *>(g.Bt + (long long)n * g.ldb + k);
+      } else {
+        uint32_t w[4];
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t lo = (n < g.N && k + 2 * e < kend)
+                                  ? g.Bt[(long long)n * g.ldb + k + 2 * e] : 0u;
+          const uint32_t hi = (n < g.N && k + 2 * e + 1 < kend)
+                                  ? g.Bt[(long long)n * g.ldb + k + 2 * e + 1] : 0u;
+          w[e] = lo | (hi << 16);
+        }
+        rb[q] = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = q * 256 + tid, r = idx >> 3, kq = (idx & 7) * 4;
+      *reinterpret_cast<uint2 *>(&As[buf][r][kq]) =
+          make_uint2(pack_bf16x2(ra[q].x, ra[q].y), pack_bf16x2(ra[q].z, ra[q].w));
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int idx = q * 256 + tid, r = idx >> 2, k8 = (idx & 3) * 8;
+      *reinterpret_cast<uint4 *>(&Bs[buf][r][k8]) = rb[q];
+    }
+  };
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  int buf = 0;
+  if (kbeg < kend) {
+    gload(kbeg);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += TBK) {
+    const bool more = k0 + TBK < kend;
+    if (more) gload(k0 + TBK);
+#pragma unroll
+    for (int kk = 0; kk < TBK; kk += 32) {
+      bf16x8 af[2], bfr[4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        af[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][wm + i * 16 + fr][kk + fk]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][wn + j * 16 + fr][kk + fk]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn + j * 16 + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        float v = acc[i][j][r];
+        if (g.epi == EPI_PARTIAL) {
+          g.C[((long long)blockIdx.z * g.M + m) * g.N + n] = v;
+          continue;
+        }
+        if (g.epi == EPI_BIAS || g.epi == EPI_BIAS_GELU) v += g.bias[n];
+        if (g.epi == EPI_BIAS_GELU) {
+          if (g.Z) g.Z[(long long)m * g.N + n] = v;
+          v = gelu(v);
+        }
+        g.C[(long long)m * g.N + n] = v;
+      }
+}
+
+// BF16 copies of one K x N weight matrix W (row-major): Wb (same layout) and WbT (N x K),
+// through 64 x 64 LDS tiles so both writes are coalesced.
+__global__ void __launch_bounds__(256) k_weights_bf16(const float *__restrict__ W,
+                                                       uint16_t *__restrict__ Wb,
+                                                       uint16_t *__restrict__ WbT, int K,
+                                                       int N) {
+  __shared__ uint16_t t[64][66];
+  const int k0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int r = i >> 6, c = i & 63, k = k0 + r, n = n0 + c;
+    if (k < K && n < N) {
+      const uint16_t v = f2bf(W[(long long)k * N + n]);
+      Wb[(long long)k * N + n] = v;
+      t[r][c] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int r = i >> 6, c = i & 63, n = n0 + r, k = k0 + c;
+    if (k < K && n < N) WbT[(long long)n * K + k] = t[c][r];
+  }
+}
+
 // C[m][n] = sum over s of P[s][m][n] in order s = 0, 1, ...
 __global__ void k_sum_splits(const float *P, float *C, long long mn, int splits) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < mn;
@@ -217,6 +375,7 @@ struct tbec_mlp {
   std::vector<long long> woff, boff;  // parameter offsets per layer
   long long nparam = 0;
   float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *wb = nullptr, *wbt = nullptr;   // BF16 weights: W's layout and transposed
   int cap_batch = 0;
   std::vector<float *> H, Z;          // activations per layer (H[0] = input)
   float *dA = nullptr, *dB = nullptr; // gradient scratch (max layer width x batch)
@@ -265,6 +424,60 @@ int gemm(tbec_mlp *m, const float *A, long long sam, long long sak, const float 
   return 0;
 }
 
+// C = A * B with A FP32 row-major (lda) and B given as BF16 Bt[n][k] (ldb); split over K
+// as gemm() does when the output has few tiles.
+int gemm_bt(tbec_mlp *m, const float *A, long long lda, const uint16_t *Bt, long long ldb,
+            float *C, int M, int N, int K, int epi, const float *bias = nullptr,
+            float *Z = nullptr) {
+  GemmBtArgs g{A, lda, Bt, ldb, C, Z, bias, M, N, K, 1, epi};
+  const long long tiles = (long long)((M + TBM - 1) / TBM) * ((N + TBN - 1) / TBN);
+  int splits = 1;
+  // few output tiles over a long K (dH of the wide output layer): split K until about
+  // 512 workgroups stream the weights, each over at least 4 K-steps
+  if (epi == EPI_STORE && tiles < 128 && K >= 8 * TBK) {
+    splits = (int)std::min<long long>(K / (4 * TBK), 512 / std::max<long long>(tiles, 1));
+    splits = std::max(1, splits);
+  }
+  if (splits > 1) {
+    const long long need = (long long)splits * M * N;
+    if (need > m->split_elems) {
+      if (m->split) (void)hipFree(m->split);
+      m->split = nullptr;
+      MCHK(hipMalloc(&m->split, sizeof(float) * need));
+      m->split_elems = need;
+    }
+    g.C = m->split;
+    g.epi = EPI_PARTIAL;
+    g.ksplit = splits;
+  }
+  dim3 grid((N + TBN - 1) / TBN, (M + TBM - 1) / TBM, splits);
+  hipLaunchKernelGGL(k_gemm_bt, grid, dim3(256), 0, m->stream, g);
+  MCHK(hipGetLastError());
+  if (splits > 1) {
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk((long long)M * N)), dim3(256), 0, m->stream,
+                       m->split, C, (long long)M * N, splits);
+    MCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+// k_gemm_bt needs both operands contiguous in k with 16-byte aligned rows.
+bool bt_ok(const float *A, long long lda, long long ldb) {
+  return ((uintptr_t)A & 15) == 0 && (lda & 3) == 0 && (ldb & 7) == 0;
+}
+
+// Refresh the BF16 weight copies from the FP32 master weights (after every update).
+int refresh_bf16(tbec_mlp *m) {
+  for (size_t l = 0; l + 1 < m->dims.size(); ++l) {
+    const int K = m->dims[l], N = m->dims[l + 1];
+    dim3 grid((N + 63) / 64, (K + 63) / 64);
+    hipLaunchKernelGGL(k_weights_bf16, grid, dim3(256), 0, m->stream, m->params + m->woff[l],
+                       m->wb + m->woff[l], m->wbt + m->woff[l], K, N);
+  }
+  MCHK(hipGetLastError());
+  return 0;
+}
+
 int ensure_batch(tbec_mlp *m, int B) {
   if (B <= m->cap_batch) return 0;
   for (float *p : m->H) if (p) (void)hipFree(p);
@@ -302,8 +515,13 @@ int forward(tbec_mlp *m, int B) {
     const float *W = m->params + m->woff[l], *b = m->params + m->boff[l];
     const bool last = l == L - 1;
     float *out = last ? m->Y : m->H[l + 1];
-    int rc = gemm(m, m->H[l], K, 1, W, N, 1, out, B, N, K,
-                  last ? EPI_BIAS : EPI_BIAS_GELU, b, last ? nullptr : m->Z[l]);
+    const int epi = last ? EPI_BIAS : EPI_BIAS_GELU;
+    int rc;
+    if (K >= TBK && bt_ok(m->H[l], K, K))   // B[k][n] = W[k][n] = WbT[n][k]
+      rc = gemm_bt(m, m->H[l], K, m->wbt + m->woff[l], K, out, B, N, K, epi, b,
+                   last ? nullptr : m->Z[l]);
+    else
+      rc = gemm(m, m->H[l], K, 1, W, N, 1, out, B, N, K, epi, b, last ? nullptr : m->Z[l]);
     if (rc) return rc;
   }
   return 0;
@@ -330,7 +548,10 @@ int backward(tbec_mlp *m, int B, double *loss) {
     if (l == 0) break;
     // dH = dZ W^T  (B = W^T: B[n][k] = W[k][n]); then dZ_prev = dH * gelu'(Z_prev)
     float *dH = (dZ == m->dA) ? m->dB : m->dA;
-    rc = gemm(m, dZ, N, 1, m->params + m->woff[l], 1, N, dH, B, K, N, EPI_STORE);
+    if (N >= TBK && bt_ok(dZ, N, N))   // B[n][k] = W[k][n] = Wb[k][n]
+      rc = gemm_bt(m, dZ, N, m->wb + m->woff[l], N, dH, B, K, N, EPI_STORE);
+    else
+      rc = gemm(m, dZ, N, 1, m->params + m->woff[l], 1, N, dH, B, K, N, EPI_STORE);
     if (rc) return rc;
     hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk((long long)B * K)), dim3(256), 0, m->stream, dH,
                        m->Z[l - 1], dH, (long long)B * K);
@@ -378,8 +599,10 @@ int tbec_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, u
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(set_error(TBEC_ERR_HIP, "stream creation failed"));
   const size_t bytes = sizeof(float) * (size_t)m->nparam;
+  const size_t hbytes = sizeof(uint16_t) * (size_t)m->nparam;
   if (hipMalloc(&m->params, bytes) != hipSuccess || hipMalloc(&m->grads, bytes) != hipSuccess ||
-      hipMalloc(&m->adam_m, bytes) != hipSuccess || hipMalloc(&m->adam_v, bytes) != hipSuccess)
+      hipMalloc(&m->adam_m, bytes) != hipSuccess || hipMalloc(&m->adam_v, bytes) != hipSuccess ||
+      hipMalloc(&m->wb, hbytes) != hipSuccess || hipMalloc(&m->wbt, hbytes) != hipSuccess)
     return bail(set_error(TBEC_ERR_OOM, "MLP parameter allocation failed"));
   // He-style uniform init: W ~ U(-a, a), a = sqrt(6 / fan_in); b = 0.
   std::vector<float> h(m->nparam, 0.f);
@@ -395,6 +618,9 @@ int tbec_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, u
   if (hipMemcpy(m->params, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(m->adam_m, 0, bytes) != hipSuccess || hipMemset(m->adam_v, 0, bytes) != hipSuccess)
     return bail(set_error(TBEC_ERR_HIP, "MLP initialisation failed"));
+  if (int rc = refresh_bf16(m)) return bail(rc);
+  if (hipStreamSynchronize(m->stream) != hipSuccess)
+    return bail(set_error(TBEC_ERR_HIP, "MLP initialisation failed"));
   *out = m;
   return 0;
 }
@@ -406,6 +632,8 @@ int tbec_mlp_destroy(tbec_mlp *m) {
   for (float *p : m->Z) if (p) (void)hipFree(p);
   for (float *p : {m->params, m->grads, m->adam_m, m->adam_v, m->dA, m->dB, m->Y, m->dY, m->T,
                    m->split, m->X_all, m->T_all})
+    if (p) (void)hipFree(p);
+  for (uint16_t *p : {m->wb, m->wbt})
     if (p) (void)hipFree(p);
   if (m->part) (void)hipFree(m->part);
   if (m->h_part) (void)hipHostFree(m->h_part);
@@ -426,6 +654,9 @@ int tbec_mlp_get_params(tbec_mlp *m, float *out) {
 int tbec_mlp_set_params(tbec_mlp *m, const float *in) {
   if (!m || !in) return set_error(TBEC_ERR_ARG, "null argument");
   MCHK(hipMemcpy(m->params, in, sizeof(float) * m->nparam, hipMemcpyHostToDevice));
+  int rc = refresh_bf16(m);
+  if (rc) return rc;
+  MCHK(hipStreamSynchronize(m->stream));
   m->adam_t = 0;
   MCHK(hipMemset(m->adam_m, 0, sizeof(float) * m->nparam));
   MCHK(hipMemset(m->adam_v, 0, sizeof(float) * m->nparam));
@@ -512,6 +743,8 @@ int tbec_mlp_train(tbec_mlp *m, int32_t epochs, int32_t batch, float lr, uint64_
       hipLaunchKernelGGL(k_adam, dim3(nblk(m->nparam)), dim3(256), 0, m->stream, m->params,
                          m->grads, m->adam_m, m->adam_v, m->nparam, lr, b1, b2, eps, c1, c2);
       MCHK(hipGetLastError());
+      rc = refresh_bf16(m);
+      if (rc) return rc;
       lsum += loss;
       ++nbat;
     }

@@ -92,7 +92,9 @@ class TbecError(RuntimeError):
 
 EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi",
            "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
-           "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve", "tbec_set_norm", "tbec_sync", "tbec_stream",
+           "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve",
+           "tbec_set_residual_gate", "tbec_batch_set_residual_gate", "tbec_set_norm",
+           "tbec_sync", "tbec_stream",
            "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version",
            "tbec_batch_create", "tbec_batch_destroy", "tbec_batch_upload_psi",
            "tbec_batch_download_psi", "tbec_batch_imag_solve", "tbec_batch_set_tuning",
@@ -128,6 +130,8 @@ def load(path: str = LIB_PATH):
     L.tbec_imag_solve.argtypes = [H, C.c_int64, C.c_double, C.c_double, C.c_int,
                                   C.POINTER(TbecSolveResult)]
     L.tbec_set_norm.argtypes = [H, C.c_double]
+    L.tbec_set_residual_gate.argtypes = [H, C.c_double]
+    L.tbec_batch_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
     L.tbec_set_w_storage.argtypes = [H, C.c_int]
@@ -296,6 +300,10 @@ class Condensate:
                                         out.ctypes.data_as(C.c_void_p), cap, C.byref(cnt)))
         return out[:min(cnt.value, cap)].copy()
 
+    def set_residual_gate(self, rmax: float):
+        """GP-residual gate of the imaginary-time stop rule (0: none)."""
+        _check(load().tbec_set_residual_gate(self._h, rmax))
+
     def set_norm(self, norm: float):
         _check(load().tbec_set_norm(self._h, norm))
 
@@ -430,6 +438,9 @@ class Batch:
     def set_tuning(self, tile, seg):
         _check(load().tbec_batch_set_tuning(self._h, tile, seg))
 
+    def set_residual_gate(self, rmax):
+        _check(load().tbec_batch_set_residual_gate(self._h, rmax))
+
     def imag_solve(self, max_steps, tol, norm=1.0, check_every=50):
         res = (TbecSolveResult * self.n)()
         _check(load().tbec_batch_imag_solve(self._h, max_steps, tol, norm, check_every, res))
@@ -499,15 +510,18 @@ class MLP:
                                        y.ctypes.data_as(C.c_void_p)))
         return y
 
-    def loss_grad(self, x, t):
+    def loss_grad(self, x, t, with_grads=True):
+        """Mean squared error and (with_grads) every parameter's gradient; without, the
+        gradients stay on the device (no host copy)."""
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
-        g = np.empty(self.nparam, np.float32)
+        g = np.empty(self.nparam, np.float32) if with_grads else None
         loss = C.c_double()
         _check(load().tbec_mlp_loss_grad(self._h, x.ctypes.data_as(C.c_void_p),
                                          t.ctypes.data_as(C.c_void_p), x.shape[0],
-                                         C.byref(loss), g.ctypes.data_as(C.c_void_p)))
-        return loss.value, g
+                                         C.byref(loss),
+                                         g.ctypes.data_as(C.c_void_p) if with_grads else None))
+        return (loss.value, g) if with_grads else loss.value
 
     def set_data(self, x, t):
         self._x = np.ascontiguousarray(x, np.float32)
