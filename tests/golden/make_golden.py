@@ -168,8 +168,9 @@ def case_info(name, prec, nprocs=1, shape=(12, 10, 8), box=(12.0, 10.0, 8.0), st
               shots=5, seed=23, keep_var=False):
     """Psi_info / Psi_norm (src/diagnostic.f90:7-25) as the reference prints them every
     `shots` steps (src/run.f90:59-63): real-time RK3 from a seeded state on a ragged grid;
-    stored: psi0, the printed rows and the state of the first shot (var0001.dat).  keep_var also stores the raw
-    per-rank snapshot records (src/output.f90:71-78) for the gather test."""
+    stored: psi0, the printed rows and the state of the first shot (var0001.dat).  keep_var
+    also stores the raw per-rank snapshot records (src/output.f90:71-78) and dims.log
+    (src/output.f90:48-58) for the gather test."""
     harm = tuple(L * L for L in box)
     d = refbuild.build(name, prec=prec, Nx=shape[0], Ny=shape[1], Nz=shape[2],
                        Lx=box[0], Ly=box[1], Lz=box[2], nsteps=steps, shots=shots,
@@ -180,6 +181,8 @@ def case_info(name, prec, nprocs=1, shape=(12, 10, 8), box=(12.0, 10.0, 8.0), st
     assert rows.shape == (steps // shots, 6), (rows.shape, r["stdout"][-2000:])
     extra = {}
     if keep_var:
+        with open(os.path.join(d, "data", "dims.log"), "rb") as fh:
+            extra["raw_dims_log"] = np.frombuffer(fh.read(), np.uint8)
         for rk in range(nprocs):
             for shot in range(1, steps // shots + 1):
                 fn = os.path.join(d, "data", f"proc{rk:03d}", f"var{shot:04d}.dat")

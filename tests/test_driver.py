@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, oracle
+from conftest import GOLDEN, ROOT, oracle
 from trapbec_amd import initial
 
 DRV = os.path.join(ROOT, "trapbec_amd", "driver", "bin", "tbec_run")
@@ -101,6 +101,45 @@ def test_driver_run_matches_oracle(tmp_path):
     s.rk3(20)
     assert np.abs(psi - s.interior).max() < 1e-12 * np.abs(s.interior).max()
     assert hdr[3] == pytest.approx(0.02, rel=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_reference_imaginary_stage(tmp_path):
+    """The reference's imaginary stage through the driver (imag_scheme='euler',
+    src/run.f90:19-33): 20 unnormalised Euler steps from the reference's own soliton state,
+    shots = 10.  The imag_var0002.dat snapshot (src/output.f90:62-69) equals the
+    reference's own imag_var0001.dat of a 20-step, shots = 20 run (the euler_f64_16
+    fixture) to 1e-12, and the printed norm ratios intPsi2/intPsi2_old of steps 10 and 20
+    equal the oracle's to 1e-11."""
+    _need_driver()
+    f = np.load(os.path.join(GOLDEN, "euler_f64_16.npz"))
+    box = tuple(float(b) for b in f["box"])
+    harm = tuple(float(h) for h in f["harm_w"])
+    nml = tmp_path / "imag.nml"
+    _nml(nml, Nx=16, Ny=16, Nz=16, Lx=box[0], Ly=box[1], Lz=box[2], harm_wx=harm[0],
+         harm_wy=harm[1], harm_wz=harm[2], dt_re=float(f["dt"]), init_cond="soliton",
+         imaginary_time=True, imag_scheme="euler", imag_nsteps=20, shots=10,
+         imaginary_output=True, nsteps=0, out_dir=str(tmp_path / "data"))
+    print("\n[case] driver reference imaginary stage 16^3", flush=True)
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, hdr, psi = _read_var(tmp_path / "data" / "proc000" / "imag_var0002.dat", (16, 16, 16))
+    ref = f["psi"]
+    assert np.abs(psi - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert hdr[3] == 0.0                       # imaginary time does not advance t
+    rows = [ln.split() for ln in r.stdout.splitlines()]
+    ratios = [float(t[1]) for t in rows if len(t) == 2 and t[0] in ("1", "2")]
+    psi0 = initial.reference_soliton((16, 16, 16), box, harm)
+    s = oracle.Stepper(oracle.params((16, 16, 16), box, harm, dt=float(f["dt"])), psi0)
+    want = []
+    for _ in range(2):
+        s.euler(9)
+        n_old = float(np.sum(np.abs(s.interior) ** 2))
+        s.euler(1)
+        want.append(float(np.sum(np.abs(s.interior) ** 2)) / n_old)
+    assert len(ratios) == 2
+    assert ratios == pytest.approx(want, rel=1e-11)
 
 
 @pytest.mark.gpu

@@ -52,6 +52,39 @@ def test_gather_four_ranks(tmp_path, prec):
         assert np.array_equal(rs["psi"], psi[4 * cx:4 * cx + 4, 3 * cy:3 * cy + 3])
 
 
+def test_gather_reference_four_rank_output(tmp_path):
+    """varfile.gather on files the reference itself wrote: a 4-rank (2 x 2) run of
+    12 x 10 x 8 FP64, its dims.log (src/output.f90:48-58) and every rank's var000N.dat
+    (src/output.f90:71-78), as bin/gather.m assembles them.  Shot 1 equals the oracle's
+    state after its 5 steps bit for bit (decomposition invariance), every shot carries
+    the reference's t and L, and shots 2-4 differ from shot 1."""
+    from conftest import GOLDEN, oracle
+    f = np.load(os.path.join(GOLDEN, "info_f64_ragged_np4.npz"))
+    d = tmp_path / "data"
+    d.mkdir()
+    (d / "dims.log").write_bytes(f["raw_dims_log"].tobytes())
+    for key in f.files:
+        if key.startswith("raw_proc"):
+            rank, shot = key[4:11], key[12:]
+            os.makedirs(d / rank, exist_ok=True)
+            (d / rank / f"{shot}.dat").write_bytes(f[key].tobytes())
+    assert varfile.read_dims(d) == {"Nx": 12, "Ny": 10, "Nz": 8, "nprocx": 2, "nprocy": 2,
+                                    "nmeshx": 6, "nmeshy": 5, "nmeshz": 8}
+    box = tuple(float(b) for b in f["box"])
+    s = oracle.Stepper(oracle.params((12, 10, 8), box, tuple(float(h) for h in f["harm_w"]),
+                                     dt=1e-3), f["psi0"])
+    shots = int(f["shots"])
+    prev = None
+    for shot in range(1, int(f["steps"]) // shots + 1):
+        g = varfile.gather(d, shot, 64)
+        s.rk3(shots)
+        assert g["L"] == box and g["t"] == pytest.approx(s.t, rel=1e-15)
+        assert np.array_equal(g["psi"], s.interior), shot
+        if prev is not None:
+            assert not np.array_equal(g["psi"], prev)
+        prev = g["psi"]
+
+
 @pytest.mark.gpu
 def test_driver_output_gathers(tmp_path):
     """The driver's dims.log and var0001.dat load through gather (one domain)."""

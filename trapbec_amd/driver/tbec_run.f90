@@ -52,6 +52,11 @@ program tbec_run
   character(len=256) :: trap_file = ''
   integer :: imag_nsteps = 0, imag_check = 50
   real(dp) :: imag_tol = 1.0e-10_dp, imag_norm = 1.0_dp
+  ! imaginary-time scheme: 'rk3' (renormalised RK3 to imag_tol, the default) or 'euler'
+  ! (the reference's own unnormalised Euler for imag_nsteps steps, src/run.f90:19-33);
+  ! imaginary_output writes imag_var%04d.dat snapshots (src/output.f90:62-69)
+  character(len=8) :: imag_scheme = 'rk3'
+  logical :: imaginary_output = .false.
   character(len=256) :: out_dir = 'data'
   namelist /tbec_run/ Nx, Ny, Nz, Lx, Ly, Lz, dt_re, dt_im, gamma, nsteps, shots, &
     harm_wx, harm_wy, harm_wz, omega_z, c0, gnl, prec, sum_mode, tile, init_cond, &
@@ -60,7 +65,8 @@ program tbec_run
     full_binary_output, imag_nsteps, imag_check, imag_tol, imag_norm, out_dir, &
     vel_print, qpress_print, boundaries, periodicz, potential, lbeam, wbeam, &
     potential_off, t_spindown, advective_vx, forcing, force_freq, trap_file, two_dim, &
-    winding_number, vor_ring_count, ring_rad, add_pcurrent, add_ring, add_dipole
+    winding_number, vor_ring_count, ring_rad, add_pcurrent, add_ring, add_dipole, &
+    imag_scheme, imaginary_output
 
   character(len=512) :: nml_file
   integer :: u, ios, itime, nstart, l
@@ -90,6 +96,8 @@ program tbec_run
   if (prec /= 32 .and. prec /= 64) call emergency_stop('prec must be 32 or 64')
   if (shots < 1) call emergency_stop('shots must be positive')
   if (forcing .and. force_freq < 1) call emergency_stop('force_freq must be positive')
+  if (trim(imag_scheme) /= 'rk3' .and. trim(imag_scheme) /= 'euler') &
+    call emergency_stop('imag_scheme must be rk3 or euler')
   l = pot_kind()   ! a wrong name stops here, before the device is touched
   if (sum_mode < 0) then
     sum_mode = TBEC_SUM_REFERENCE
@@ -178,8 +186,12 @@ program tbec_run
   end if
 
   if (imaginary_time .and. nstart == 1) then
-    ! the imaginary-time stage needs dt = (0, -dtau): run it on a second handle
-    call imaginary_stage()
+    if (trim(imag_scheme) == 'euler') then
+      call imaginary_stage_euler()
+    else
+      ! renormalised RK3 needs dt = (0, -dtau): run it on a second handle
+      call imaginary_stage()
+    end if
   end if
   ! src/run.f90:37-40
   if (add_pcurrent) call check(tbec_reference_insert(h, TBEC_INSERT_PCURRENT, ric))
@@ -277,28 +289,72 @@ contains
     end if
     call check(tbec_destroy(hi))
     call upload()
+    ! one snapshot of the solve's final state, numbered by the shots it spans
+    if (imaginary_output) call write_snapshot('imag_var', int((sr%steps + shots - 1) / shots))
     write(*, '(a)') ' finished imaginary time propagation'
   end subroutine
 
-  ! var%04d.dat: one record coords(2), Lx, Ly, Lz, t, Psi; var.dat: [itime+1] [t] [Psi]
-  ! (src/output.f90:75-77,103-107), written in the run's precision.
-  subroutine write_var(shot)
+  ! The reference's imaginary stage (src/run.f90:19-33): imag_nsteps unnormalised steps
+  ! psi += dt*0.5*rhs on the main handle (the run's own dt); every `shots` steps the
+  ! norm ratio of the last step, intPsi2/intPsi2_old (Psi_norm's sum of |psi|^2, the old
+  ! value 1 before the first step), and with imaginary_output the imag_var snapshot.
+  subroutine imaginary_stage_euler()
+    integer :: it, n
+    real(dp) :: n_old, n_new
+    it = 0
+    n_old = 1.0_dp
+    do while (it < imag_nsteps)
+      n = min(shots - mod(it, shots), imag_nsteps - it)
+      if (n > 1) then
+        call check(tbec_step(h, int(n - 1, c_int64_t), TBEC_IMAG_EULER_REF, t))
+        it = it + n - 1
+        call check(tbec_diagnose(h, d))
+        n_old = d%int_psi2
+      end if
+      call check(tbec_step(h, 1_c_int64_t, TBEC_IMAG_EULER_REF, t))
+      it = it + 1
+      call check(tbec_diagnose(h, d))
+      n_new = d%int_psi2
+      if (mod(it, shots) == 0) then
+        print *, it / shots, n_new / n_old
+        if (imaginary_output) then
+          call download()
+          call write_snapshot('imag_var', it / shots)
+        end if
+      end if
+      n_old = n_new
+    end do
+    call download()
+    write(*, '(a)') ' finished imaginary time propagation'
+  end subroutine
+
+  ! <prefix>%04d.dat: one record coords(2), Lx, Ly, Lz, t, Psi (src/output.f90:65-67,
+  ! 75-77), in the run's precision.
+  subroutine write_snapshot(prefix, shot)
+    character(len=*), intent(in) :: prefix
     integer, intent(in) :: shot
     character(len=512) :: fname
     integer :: uv
     integer(4) :: crd(2)
     crd = 0
-    if (full_binary_output) then
-      write(fname, '(a,a,i4.4,a)') trim(out_dir), '/proc000/var', shot, '.dat'
-      open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
-      if (prec == 64) then
-        write(uv) crd, Lx, Ly, Lz, real(t, dp), psi
-      else
-        write(uv) crd, real(Lx, real32), real(Ly, real32), real(Lz, real32), &
-          real(t, real32), cmplx(psi, kind=real32)
-      end if
-      close(uv)
+    write(fname, '(a,a,a,i4.4,a)') trim(out_dir), '/proc000/', prefix, shot, '.dat'
+    open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
+    if (prec == 64) then
+      write(uv) crd, Lx, Ly, Lz, real(t, dp), psi
+    else
+      write(uv) crd, real(Lx, real32), real(Ly, real32), real(Lz, real32), &
+        real(t, real32), cmplx(psi, kind=real32)
     end if
+    close(uv)
+  end subroutine
+
+  ! var%04d.dat (write_snapshot); var.dat: [itime+1] [t] [Psi] (src/output.f90:103-107),
+  ! written in the run's precision.
+  subroutine write_var(shot)
+    integer, intent(in) :: shot
+    character(len=512) :: fname
+    integer :: uv
+    if (full_binary_output) call write_snapshot('var', shot)
     write(fname, '(a,a)') trim(out_dir), '/proc000/var.dat'
     open(newunit=uv, file=trim(fname), status='replace', form='unformatted')
     write(uv) int(itime + 1, 4)
