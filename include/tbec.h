@@ -302,7 +302,11 @@ int tbec_imprint_vortex(tbec_handle *h, double x0, double y0, double dir);
  * rebuild.  Drops captured graphs. */
 int tbec_set_w_storage(tbec_handle *h, int stored);
 /* Tile order: 1 (default) gives each of the 8 XCDs a contiguous block of tiles so that
- * neighbouring tiles share that XCD's L2; 0 keeps the dispatch order. */
+ * neighbouring tiles share that XCD's L2; 0 keeps the dispatch order; 2
+ * (TBEC_REMAP_FIRST_STAGE) uses the contiguous order only for the stages that read one
+ * array (the first RK stage, the reference's Euler step, the rhs pass) and the dispatch
+ * order for the RK stages that also read a pointwise array. */
+enum { TBEC_REMAP_FIRST_STAGE = 2 };
 int tbec_set_xcd_remap(tbec_handle *h, int on);
 
 int tbec_sync(tbec_handle *h);

@@ -14,7 +14,7 @@ def test_tune_records_the_fastest_candidate(tmp_path):
     table = str(tmp_path / "tuning.json")
     print(f"\n[case] autotune {shape} fp{prec}", flush=True)
     e = autotune.tune(shape, prec, nstages=6, table=table)
-    assert len(e["all"]) == 2 * tbec.NUM_TILES * len(autotune.SEGS)
+    assert len(e["all"]) == 3 * tbec.NUM_TILES * len(autotune.SEGS)
     best = min(a[3] for a in e["all"])
     assert round(e["us_per_stage"], 2) == best
     assert [e["tile"], e["seg"], e["remap"]] == e["all"][0][:3]

@@ -1,10 +1,12 @@
 """Tile-shape autotuner for the fused stage kernel.
 
 The stage kernel is compiled for five (TZ, TY, VZ) tile shapes per precision and takes
-the x-segment length SX at run time.  `tune` times every (tile, SX) candidate on the
-GPU for one grid shape with HIP events (libtbec's tbec_time_stages) and records the
-fastest in trapbec_amd/tuning.json; `lookup` returns the recorded choice, else a
-default tile with an automatic segment length (seg 0).
+the x-segment length SX and the tile order over the XCDs (dispatch order, contiguous
+blocks per XCD, or contiguous for the first RK stage only) at run time.  `tune` times
+every (tile, SX, order) candidate on the GPU for one grid shape with HIP events
+(libtbec's tbec_time_stages) and records the fastest in trapbec_amd/tuning.json;
+`lookup` returns the recorded choice, else a default tile with an automatic segment
+length (seg 0).
 """
 from __future__ import annotations
 
@@ -66,7 +68,7 @@ def tune(shape, prec, sum_mode=None, psi0=None, nstages=30, save=True, verbose=T
         psi0 = np.full(shape, 0.1 + 0.05j)
     c.upload(psi0)
     results = []
-    for remap in (1, 0):
+    for remap in (1, 0, 2):
         c.set_xcd_remap(remap)
         for tile in range(tbec.NUM_TILES):
             for seg in SEGS:

@@ -373,7 +373,9 @@ static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A,
   a.w = h->w;
   a.cre = cre; a.cim = cim; a.A = A; a.B = B;
   a.stage = stage;
-  a.remap = h->remap;
+  // TBEC_REMAP_FIRST_STAGE: the XCD-contiguous order for the stages that read one array
+  // (the first RK stage, Euler, rhs-only), the dispatch order for the two-array stages
+  a.remap = h->remap == 1 ? 1 : (h->remap == 2 && stage != ST_RK2 && stage != ST_RK3) ? 1 : 0;
   return a.nblocks;
 }
 
@@ -708,7 +710,8 @@ int tbec_set_xcd_remap(tbec_handle *h, int on) {
   if (!h) return fail(TBEC_ERR_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
   tbec_drop_graphs(h);
-  h->remap = on ? 1 : 0;
+  if (on < 0 || on > 2) return fail(TBEC_ERR_ARG, "xcd remap must be 0, 1 or 2");
+  h->remap = on;
   return 0;
 }
 

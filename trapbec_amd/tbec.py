@@ -310,8 +310,10 @@ class Condensate:
     def set_tuning(self, tile: int, seg: int):
         _check(load().tbec_set_tuning(self._h, tile, seg))
 
-    def set_xcd_remap(self, on: bool):
-        _check(load().tbec_set_xcd_remap(self._h, int(bool(on))))
+    def set_xcd_remap(self, on: int):
+        """0 dispatch order, 1 XCD-contiguous tile order for every stage, 2 for the
+        first RK stage only (TBEC_REMAP_FIRST_STAGE)."""
+        _check(load().tbec_set_xcd_remap(self._h, int(on)))
 
     def set_boundaries(self, boundaries="zero", periodicz: bool = False):
         """Boundary faces as the reference's ghost_comm fills them: "zero", "solid"
