@@ -22,8 +22,9 @@ stage 3 reads psi2 and psi1, no w) the kernels need only 8 (64 B / 128 B);
 `roofline.frac_of_bytes_moved` states the fraction on that basis.
 
 `stream_ceiling` is the same step's read/write mix moved by plain streaming kernels
-(torch's copy and add over n^3 complex arrays, timed here): the rate HBM actually delivers
-for that mix on this GPU, and `stream_ceiling.frac` the stepper's value against it.
+(a copy and an add over n^3 complex arrays, the faster of torch's and libtbec's
+hand-written non-temporal ones, timed here): the rate HBM actually delivers for that mix
+on this GPU, and `stream_ceiling.frac` the stepper's value against it.
 """
 from __future__ import annotations
 
@@ -160,12 +161,15 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False, table=Fa
 
 
 def stream_ceiling(n, prec, reps=20):
-    """The read/write mix of one RK3 step with w rebuilt, streamed by torch's own
-    elementwise kernels over n^3 complex arrays: stage 1 reads one array and writes one (a
-    copy), stages 2 and 3 read two and write one (an add).  copy + 2 x add is the time a
-    step would take if its stages moved only their algorithmic bytes at the rate plain
-    streaming kernels reach on this GPU.  Timed with events on torch's stream."""
+    """The read/write mix of one RK3 step with w rebuilt, moved by plain streaming kernels
+    over n^3 complex arrays: stage 1 reads one array and writes one (a copy), stages 2
+    and 3 read two and write one (an add).  copy + 2 x add is the time a step would take
+    if its stages moved only their algorithmic bytes at the rate plain streaming reaches
+    on this GPU.  Each operation is timed with torch's own kernels (events on torch's
+    stream) and with libtbec's hand-written 16-byte non-temporal ones
+    (tbec_stream_rates); the faster of the two sets the ceiling."""
     import torch
+    from trapbec_amd import tbec
     dt = torch.complex64 if prec == 32 else torch.complex128
     x = torch.ones(n ** 3, dtype=dt, device="cuda:0")
     y = torch.ones_like(x)
@@ -183,17 +187,22 @@ def stream_ceiling(n, prec, reps=20):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    copy_ms = timed(lambda: z.copy_(x))
-    add_ms = timed(lambda: torch.add(x, y, out=z))
+    t_copy = timed(lambda: z.copy_(x))
+    t_add = timed(lambda: torch.add(x, y, out=z))
     del x, y, z
     torch.cuda.empty_cache()
-    step_ms = copy_ms + 2.0 * add_ms
     esize = 8 if prec == 32 else 16
+    h_copy_us, h_add_us = tbec.stream_rates(esize * n ** 3, reps)
+    copy_ms = min(t_copy, h_copy_us * 1e-3)
+    add_ms = min(t_add, h_add_us * 1e-3)
+    step_ms = copy_ms + 2.0 * add_ms
     return {"copy_us": round(copy_ms * 1e3, 1), "add_us": round(add_ms * 1e3, 1),
             "copy_GBps": round(2 * esize * n ** 3 / (copy_ms * 1e-3) / 1e9, 1),
             "add_GBps": round(3 * esize * n ** 3 / (add_ms * 1e-3) / 1e9, 1),
             "step_ms": round(step_ms, 5),
-            "value": round(n ** 3 / (step_ms * 1e-3) / 1e9, 4)}
+            "value": round(n ** 3 / (step_ms * 1e-3) / 1e9, 4),
+            "torch_us": {"copy": round(t_copy * 1e3, 1), "add": round(t_add * 1e3, 1)},
+            "handwritten_us": {"copy": round(h_copy_us, 1), "add": round(h_add_us, 1)}}
 
 
 def host_cores():
@@ -632,8 +641,9 @@ def main():
                 "tile": v["tile"], "seg": v["seg"], "steps": steps}
     sc = res["stream"][head]
     out["stream_ceiling"] = dict(sc, frac=round(out["value"] / sc["value"], 4),
-                                 basis="torch copy (stage 1) + 2 x torch add (stages 2, 3) "
-                                       f"over {n}^3 complex arrays")
+                                 basis="copy (stage 1) + 2 x add (stages 2, 3) over "
+                                       f"{n}^3 complex arrays, the faster of torch's and "
+                                       "hand-written non-temporal kernels for each")
     if 64 in res and head == 32:
         out["fp64"] = brief(res[64], args.fp64_steps)
         out["fp64"]["stream_ceiling"] = dict(res["stream"][64], frac=round(

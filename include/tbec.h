@@ -309,6 +309,12 @@ int tbec_set_w_storage(tbec_handle *h, int stored);
 enum { TBEC_REMAP_FIRST_STAGE = 2 };
 int tbec_set_xcd_remap(tbec_handle *h, int on);
 
+/* Plain streaming rates over arrays of `bytes` bytes (allocated and freed here): the mean
+ * time of a copy (one array read, one written) and of an add (two read, one written),
+ * 16-byte accesses with non-temporal writes, over `reps` launches each.  The streaming
+ * ceiling of the stepper's read/write mix.  No counterpart in the reference. */
+int tbec_stream_rates(int64_t bytes, int reps, double *copy_us, double *add_us);
+
 int tbec_sync(tbec_handle *h);
 /* The handle's HIP stream (hipStream_t), for event timing by the caller. */
 void *tbec_stream(tbec_handle *h);

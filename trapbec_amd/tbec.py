@@ -94,7 +94,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
            "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve",
            "tbec_set_residual_gate", "tbec_batch_set_residual_gate", "tbec_set_norm",
-           "tbec_sync", "tbec_stream",
+           "tbec_sync", "tbec_stream", "tbec_stream_rates",
            "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version",
            "tbec_batch_create", "tbec_batch_destroy", "tbec_batch_upload_psi",
            "tbec_batch_download_psi", "tbec_batch_imag_solve", "tbec_batch_set_tuning",
@@ -130,6 +130,8 @@ def load(path: str = LIB_PATH):
     L.tbec_imag_solve.argtypes = [H, C.c_int64, C.c_double, C.c_double, C.c_int,
                                   C.POINTER(TbecSolveResult)]
     L.tbec_set_norm.argtypes = [H, C.c_double]
+    L.tbec_stream_rates.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
     L.tbec_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_batch_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
@@ -536,6 +538,13 @@ class MLP:
         _check(load().tbec_mlp_train(self._h, epochs, batch, lr, seed,
                                      hist.ctypes.data_as(C.c_void_p)))
         return hist
+
+
+def stream_rates(nbytes: int, reps: int = 20):
+    """(copy_us, add_us): plain streaming kernels over arrays of nbytes bytes."""
+    cu, au = C.c_double(), C.c_double()
+    _check(load().tbec_stream_rates(int(nbytes), int(reps), C.byref(cu), C.byref(au)))
+    return cu.value, au.value
 
 
 def upload_coarse(cond: Condensate, coarse: np.ndarray):
