@@ -101,7 +101,9 @@ int tbec_download_psi(tbec_handle *h, void *psi_host);
 int tbec_upload_psi_device(tbec_handle *h, const void *psi_dev);
 int tbec_download_psi_device(tbec_handle *h, void *psi_dev);
 
-/* Advance nsteps steps of the given mode (enqueued; returns before completion).
+/* Advance nsteps steps of the given mode (enqueued; returns before completion, except
+ * TBEC_IMAG_RK3, which synchronises to read the finalisers' flag and returns
+ * TBEC_ERR_NONFINITE for a non-finite norm or energy).
  * Real time advances the handle's time by Re(dt) per step in the working precision
  * (src/run.f90:55); t_inout, when not null, sets that time first and receives it after.
  * Each step evaluates the potential, the frame-term switch and the switch-off at the

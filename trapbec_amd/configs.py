@@ -78,9 +78,9 @@ def run_c2(n=256, steps=2000):
     return out
 
 
-def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
+def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200, gate=0.0):
     pr, psi0 = initial.config_c3(n=n)
-    out = {"config": "c3", "grid": n, "tol": tol}
+    out = {"config": "c3", "grid": n, "tol": tol, "residual_gate": gate}
     res = {}
     from trapbec_amd import autotune
     for prec in (64, 32):
@@ -88,6 +88,7 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
         with tbec.Condensate.from_problem(pr, prec=prec) as c:
             c.set_tuning(e["tile"], e["seg"])
             c.set_xcd_remap(e["remap"])
+            c.set_residual_gate(gate)
             c.upload(psi0)
             print(f"[c3] fp{prec} imaginary-time solve {pr.shape}, tile {e['tile']} seg "
                   f"{e['seg']} remap {e['remap']}: {e['us_per_stage']:.1f} us/stage",
@@ -113,10 +114,13 @@ def run_c3(n=256, tol=1e-10, max_steps=200000, check_every=200):
     return out
 
 
-def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
+def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32, gate=0.0):
+    """gate > 0: the stop rule also requires the GP residual at or below it (a rotating
+    instance cannot stop in its vortex-free saddle)."""
     from trapbec_amd import autotune
     probs = initial.config_c4(n_inst=n_inst)
     out = {"config": "c4", "instances": n_inst, "tol": tol, "max_steps": max_steps,
+           "residual_gate": gate,
            "total_points": int(sum(np.prod(p.shape) for p, _ in probs))}
     # tile shapes from the autotuner: one (tile, seg) for the ensemble, one per grid
     # shape for the one-at-a-time solves
@@ -135,14 +139,28 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
     t0 = time.time()
     with tbec.Batch([p for p, _ in probs], prec=prec) as b:
         b.set_tuning(tb["tile"], tb["seg"])
+        b.set_residual_gate(gate)
         for i, (_, psi0) in enumerate(probs):
             b.upload(i, psi0)
         t1 = time.time()
         res_b = b.imag_solve(max_steps, tol, 1.0, check_every)
         t_batch = time.time() - t1
+        fin = [b.download(i) for i in range(len(probs))]
     out["batched"] = {"wall_s": t_batch, "setup_s": t1 - t0,
                       "converged": int(sum(r["converged"] for r in res_b)),
-                      "steps": [int(r["steps"]) for r in res_b]}
+                      "steps": [int(r["steps"]) for r in res_b],
+                      "energy": [r["energy"] for r in res_b],
+                      "residual": [r["residual"] for r in res_b]}
+    # vortex cores in the midplane of each final state (z-normal plaquettes)
+    cores = []
+    for (pr, _), psi in zip(probs, fin):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.upload(psi)
+            v = c.vortex_cores(1e-5 * float(np.max(np.abs(psi)) ** 2), 0)
+            mid = pr.shape[2] // 2
+            cores.append(int(np.sum(v[:, 2] == mid)) if len(v) else 0)
+    out["batched"]["midplane_cores"] = cores
+    del fin
     print(f"[c4] batched: {t_batch:.1f} s, {out['batched']['converged']}/{n_inst} converged",
           flush=True)
     seq = []
@@ -152,6 +170,7 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32):
             tile, seg, remap = single[tuple(pr.shape)]
             c.set_tuning(tile, seg)
             c.set_xcd_remap(remap)
+            c.set_residual_gate(gate)
             c.upload(psi0)
             c.sync()
             t1 = time.time()
@@ -178,14 +197,16 @@ def main():
     ap.add_argument("--out", default="gpurun_out/configs")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-steps", type=int, default=None)
+    ap.add_argument("--gate", type=float, default=0.0,
+                    help="c3, c4: GP-residual gate of the stop rule (0: none)")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     if a.config == "c2":
         r = run_c2(n=a.n)
     elif a.config == "c3":
-        r = run_c3(n=a.n, max_steps=a.max_steps or 200000)
+        r = run_c3(n=a.n, max_steps=a.max_steps or 200000, gate=a.gate)
     else:
-        r = run_c4(max_steps=a.max_steps or 20000)
+        r = run_c4(max_steps=a.max_steps or 20000, gate=a.gate)
     with open(os.path.join(a.out, f"{a.config}.json"), "w") as f:
         json.dump(r, f, indent=1)
     print(json.dumps(r), flush=True)
