@@ -708,9 +708,9 @@ int tbec_set_w_storage(tbec_handle *h, int stored) {
 
 int tbec_set_xcd_remap(tbec_handle *h, int on) {
   if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (on < 0 || on > 2) return fail(TBEC_ERR_ARG, "xcd remap must be 0, 1 or 2");
   HIPCHK(hipStreamSynchronize(h->stream));
   tbec_drop_graphs(h);
-  if (on < 0 || on > 2) return fail(TBEC_ERR_ARG, "xcd remap must be 0, 1 or 2");
   h->remap = on;
   return 0;
 }
