@@ -303,6 +303,61 @@ def first_reaching(hist, e_target, r_target=None):
     return None
 
 
+def converge_network_start(pr, coarse, tol, gate, max_steps, check_every=1000, tuning=None,
+                           infer_s=0.0):
+    """The network start refined to the stop rule: relative energy change below tol for 8
+    consecutive steps with the GP residual at or below `gate` (tbec_imag_solve with the
+    residual gate), at most max_steps.  Returns steps, wall time (network inference,
+    upsampling and the solve), energy and residual of the final state."""
+    t0 = time.time()
+    c = _start_handle(pr, coarse=coarse, tuning=tuning)
+    c.set_residual_gate(gate)
+    c.sync()
+    r = c.imag_solve(max_steps, tol, 1.0, check_every)
+    c.sync()
+    wall = time.time() - t0 + infer_s
+    d = c.diagnose()
+    c.close()
+    return {"steps": int(r["steps"]), "converged": bool(r["converged"]), "wall_s": wall,
+            "energy": d["energy"], "residual": d["residual"], "tol": tol, "gate": gate}
+
+
+def converge_main(a):
+    """Both starts to the converged state at C5's size, in two phases that fit one GPU
+    call each: `--phase network` trains the network and runs its start to the gated stop
+    rule; `--phase tf --target E R` runs the TF start until its energy and GP residual are
+    at or below E and R (the network start's final state), or --max-steps."""
+    test = np.array(a.test)
+    pr = fine_problem(test, a.fine, a.fine_dtau)
+    tuning = (2, 32, 1)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.phase == "network":
+        m, scale, info = train_network(samples=a.samples, train_n=a.train_n, epochs=a.epochs,
+                                       log=log)
+        try:
+            coarse, infer_s = predict(m, scale, test)
+        finally:
+            m.close()
+        os.environ.setdefault("TBEC_PROGRESS", "20")
+        r = converge_network_start(pr, coarse, a.tol, a.gate, a.max_steps, tuning=tuning,
+                                   infer_s=infer_s)
+        out = {"phase": "network", "network": info, "network_start": r}
+    else:
+        c = _start_handle(pr, start=tf_start(pr, 1.0, seed=0), tuning=tuning)
+        r = run_to_state(c, a.max_steps, 1000, e_target=a.target[0], r_target=a.target[1],
+                         log=log, tag="TF start")
+        c.close()
+        rec = r.pop("record")
+        out = {"phase": "tf", "target": {"energy": a.target[0], "residual": a.target[1]},
+               "tf_start": r, "record_every_10k": [x for x in rec if x[0] % 10000 == 0]}
+    out.update({"test_params": test.tolist(), "grid": a.fine, "dtau": a.fine_dtau})
+    tag = "_".join(f"{v:g}" for v in a.test)
+    os.makedirs(a.out, exist_ok=True)
+    with open(os.path.join(a.out, f"converge_{a.phase}_{tag}.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=2000)
@@ -320,7 +375,16 @@ def main():
                     help="also run both starts this many steps without the stop rule and "
                          "compare steps to equal energy along the way")
     ap.add_argument("--out", default="gpurun_out/warmstart")
+    ap.add_argument("--phase", choices=["network", "tf"], default=None,
+                    help="converged comparison in two phases (see converge_main)")
+    ap.add_argument("--gate", type=float, default=5e-4,
+                    help="GP-residual gate of the stop rule for --phase network")
+    ap.add_argument("--target", type=float, nargs=2, default=None,
+                    help="--phase tf: energy and GP residual to reach")
     a = ap.parse_args()
+    if a.phase:
+        converge_main(a)
+        return
     os.makedirs(a.out, exist_ok=True)
     t0 = time.time()
     params, X, Y, stats = generate(a.samples, n=a.train_n,
