@@ -336,7 +336,9 @@ def warm_start_leg(args):
     then, at a held-out rotating problem on args.warm_fine^3, the network's start against
     the plain TF start at equal final energy and GP residual: the network start runs
     args.warm_steps FP32 imaginary-time steps (dtau = 8e-4), and the TF start runs until
-    it reaches that energy and residual, or args.warm_cap times the steps."""
+    it reaches that energy and residual, or args.warm_cap times the steps.  Then the
+    coarse-to-fine solve to the gated stop rule from both starts (`continuation`), with
+    the plain TF run as a lower bound on one-grid imaginary time to the same state."""
     from trapbec_amd import warmstart as ws
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     m, scale, info = ws.train_network(samples=args.warm_samples, log=log)
@@ -349,12 +351,34 @@ def warm_start_leg(args):
     r = ws.equal_state_comparison(pr, coarse, args.warm_steps, args.warm_cap,
                                   tuning=(2, 32, 1), infer_s=infer_s, log=log)
     r.pop("records")
+    # the coarse-to-fine solve (128^3 -> 192^3 -> fine, gated stop rule 1e-10 with residual
+    # <= 5e-4 at the fine grid) from the network's field and from the TF start; the plain
+    # TF run above bounds how long one-grid imaginary time takes to reach that state
+    cont = {}
+    for name, field in (("network_start", coarse), ("tf_start", None)):
+        cont[name] = ws.continuation_solve(pr, coarse=field, tuning=(2, 32, 1), log=log)
+        cont[name].pop("fine_steps")
+    cont["network_start"]["wall_s"] += infer_s
+    plain = r["tf_start"]
+    cs = cont["tf_start"]
+    reached = plain["energy"] <= cs["energy"] and plain["residual"] <= cs["residual"]
+    cont["plain_tf_run"] = {"steps": plain["steps"], "wall_s": plain["wall_s"],
+                            "energy": plain["energy"], "residual": plain["residual"],
+                            "reached_continuation_state": reached}
+    if not reached:
+        for name in ("network_start", "tf_start"):
+            cont[f"speedup_wall_lower_bound_{name}"] = plain["wall_s"] / cont[name]["wall_s"]
+    log(f"[bench] continuation: network start {cont['network_start']['wall_s']:.1f} s, TF "
+        f"start {cs['wall_s']:.1f} s (E {cs['energy']:.10f} residual {cs['residual']:.2e}); "
+        f"plain TF after {plain['wall_s']:.1f} s: E {plain['energy']:.10f} residual "
+        f"{plain['residual']:.2e}")
     out = {"metric": "ground-state time, network (warm) start vs TF start, at equal final "
                      "energy and GP residual",
            "test_params": {"w_perp": WARM_TEST[0], "w_z": WARM_TEST[1], "g": WARM_TEST[2],
                            "Omega": WARM_TEST[3], "log_N": WARM_TEST[4]},
            "grid": [args.warm_fine] * 3, "box": [-12.0, 12.0], "dtau": dtau,
-           "dtype": "f32 (FP64 reductions)", "network": info, "inference_s": infer_s, **r}
+           "dtype": "f32 (FP64 reductions)", "network": info, "inference_s": infer_s, **r,
+           "continuation": cont}
     sp = r.get("speedup_steps")
     log(f"[bench] warm start: network {r['network_start']['steps']} steps "
         f"{r['network_start']['wall_s']:.1f} s; TF "

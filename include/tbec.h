@@ -369,6 +369,13 @@ int tbec_mlp_train(tbec_mlp *m, int32_t epochs, int32_t batch, float lr, uint64_
 /* Replace psi by the trilinear upsampling of a coarse (3, cn, cn, cn) field
  * (sqrt-density, cos phase, sin phase) on the same box; resets the imaginary-time state. */
 int tbec_upload_coarse(tbec_handle *h, const float *coarse, int32_t cn);
+/* Replace dst's psi by the trilinear interpolation of src's current psi (its
+ * renormalisation applied) at dst's cell centres, coordinates clamped to src's outermost
+ * cell centres; the two handles share one box, one precision and one device.  Resets
+ * dst's imaginary-time state; synchronises.  No reference counterpart (the reference
+ * solves on one grid): the step between the levels of a coarse-to-fine ground-state
+ * solve, for callers that relax a state on a coarser grid first. */
+int tbec_prolong(tbec_handle *dst, tbec_handle *src);
 
 const char *tbec_last_error(void);
 /* Library version string and the gfx target it was built for. */

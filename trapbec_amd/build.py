@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtbec.so")
 SOURCES = ["tbec.hip", "tbec_step.hip", "tbec_batch.hip", "tbec_vortex.hip", "tbec_mlp.hip",
            "tbec_ic.hip", "tbec_diag.hip", "tbec_phys.hip", "tbec_refic.hip",
-           "tbec_noalloc.hip", "tbec_stream.hip"]
+           "tbec_noalloc.hip", "tbec_stream.hip", "tbec_prolong.hip"]
 INSTANCES = [(p, t) for p in (32, 64) for t in range(5)]
 
 # Packed FP32 instructions are disabled for every HIP file: hipcc would otherwise emit

@@ -101,7 +101,7 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
-           "tbec_upload_coarse", "tbec_set_xcd_remap", "tbec_init_state",
+           "tbec_upload_coarse", "tbec_prolong", "tbec_set_xcd_remap", "tbec_init_state",
            "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries",
            "tbec_set_physics", "tbec_set_potential_table", "tbec_potential", "tbec_set_time",
            "tbec_imprint_vortex", "tbec_reference_state", "tbec_reference_insert"]
@@ -554,3 +554,11 @@ def upload_coarse(cond: Condensate, coarse: np.ndarray):
     L = load()
     L.tbec_upload_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     _check(L.tbec_upload_coarse(cond._h, c.ctypes.data_as(C.c_void_p), c.shape[1]))
+
+
+def prolong(dst: Condensate, src: Condensate):
+    """Start `dst` from the trilinear interpolation of `src`'s current psi at dst's cell
+    centres (same box, precision and device); synchronises."""
+    L = load()
+    L.tbec_prolong.argtypes = [C.c_void_p, C.c_void_p]
+    _check(L.tbec_prolong(dst._h, src._h))

@@ -7,7 +7,13 @@ produces (Sobol-sampled parameters), on the GPU by libtbec's BF16-MFMA kernels. 
 prediction is trilinearly upsampled to the fine grid and refined by FP32 imaginary time;
 the plain Thomas-Fermi start is both the fallback and the judge.
 
+continuation_solve relaxes a start on coarser grids of the same box first (larger stable
+dtau, cheaper steps) and hands each level's state to the next by tbec_prolong: the slow
+modes of a rotating ground state (a vortex lattice settling) take the same imaginary time
+on every grid, so they are cheapest to relax on a coarse one.
+
     python -m trapbec_amd.warmstart [--samples 2000] [--fine 384] [--out DIR]
+    python -m trapbec_amd.warmstart --phase continuation
 """
 from __future__ import annotations
 
@@ -303,6 +309,85 @@ def first_reaching(hist, e_target, r_target=None):
     return None
 
 
+RK3_REAL_LIMIT = 2.5127   # |z| where 1 + z + z^2/2 + z^3/6 leaves the unit disc, z < 0
+
+
+def max_rate(pr, shape=None) -> float:
+    """Upper bound of the imaginary-time operator's largest eigenvalue on `shape` (default
+    pr.shape): the 6th-order -1/2 Laplacian at the grid's highest wavenumber
+    (3.0222/h^2 per axis), the trap at the box corner, the frame term's 6th-order first
+    derivative (1.8333/h) times |Omega| and the box's largest radius, and 2 mu_TF for the
+    nonlinearity (mu_TF of the norm-1 Thomas-Fermi profile)."""
+    shape = shape or pr.shape
+    h = [L / n for L, n in zip(pr.box, shape)]
+    kin = sum(3.0222 / hh ** 2 for hh in h)
+    v = 0.5 * sum((w * L / 2) ** 2 for w, L in zip(pr.omega, pr.box))
+    rot = abs(pr.Omega) * math.hypot(pr.box[0] / 2, pr.box[1] / 2) * 1.8333 / min(h[0], h[1])
+    wbar3 = pr.omega[0] * pr.omega[1] * pr.omega[2]
+    mu = 0.5 * (15.0 * pr.g * wbar3 / (4.0 * math.pi)) ** 0.4
+    return kin + v + rot + 2.0 * mu
+
+
+def continuation_levels(pr, factors=(3, 2), safety=0.95):
+    """Coarser grids of pr's box for a coarse-to-fine ground-state solve, coarsest first:
+    shape // factor per level, each with the largest dtau the stability bound allows
+    (safety * 2.5127 / max_rate), never below pr's own dtau."""
+    dtau = abs(complex(pr.dt))
+    out = []
+    for f in factors:
+        shape = tuple(max(2, n // f) for n in pr.shape)
+        out.append((shape, max(dtau, safety * RK3_REAL_LIMIT / max_rate(pr, shape))))
+    return out
+
+
+def continuation_solve(pr, coarse=None, factors=(3, 2), tol=1e-10, gate=5e-4,
+                       coarse_gate=1e-4, max_steps=400000, check_every=1000, tuning=None,
+                       safety=0.95, keep=False, log=print):
+    """Imaginary-time ground state of pr on a ladder of grids: the coarsest level starts
+    from the network's coarse field (`coarse`, trilinearly upsampled) or, when None, from
+    the TF profile with the seeded phase noise of tf_start; each level runs the gated stop
+    rule (relative energy change below tol for 8 steps, GP residual at or below
+    coarse_gate; at pr's own grid and dtau, at or below gate) and hands its state to the
+    next by tbec_prolong.  Returns per-level steps, wall times, energies and residuals and
+    the total wall time; keep=True also returns the final handle (the caller closes it)."""
+    levels = continuation_levels(pr, factors, safety) + [(tuple(pr.shape), abs(complex(pr.dt)))]
+    t0 = time.time()
+    prev, recs = None, []
+    for k, (shape, dtau) in enumerate(levels):
+        fine = k == len(levels) - 1
+        p = initial.Problem(shape, pr.box, pr.omega, pr.g, pr.Omega, -dtau * 1j, False,
+                            dict(pr.meta))
+        t1 = time.time()
+        c = tbec.Condensate.from_problem(p, prec=32)
+        if fine and tuning is not None:
+            c.set_tuning(tuning[0], tuning[1])
+            c.set_xcd_remap(tuning[2])
+        if prev is not None:
+            tbec.prolong(c, prev)
+            prev.close()
+        elif coarse is not None:
+            tbec.upload_coarse(c, coarse)
+        else:
+            c.upload(tf_start(p, 1.0, seed=0))
+        c.set_residual_gate(gate if fine else coarse_gate)
+        r = c.imag_solve(max_steps, tol, 1.0, check_every)
+        d = c.diagnose()
+        recs.append({"shape": list(shape), "dtau": dtau, "steps": int(r["steps"]),
+                     "converged": bool(r["converged"]), "wall_s": time.time() - t1,
+                     "energy": d["energy"], "residual": d["residual"]})
+        log(f"[continuation] {'x'.join(map(str, shape))} dtau {dtau:.3g}: "
+            f"{int(r['steps'])} steps {recs[-1]['wall_s']:.1f} s E {d['energy']:.10f} "
+            f"residual {d['residual']:.3e}{'' if r['converged'] else ' (step cap)'}")
+        prev = c
+    out = {"levels": recs, "wall_s": time.time() - t0, "energy": recs[-1]["energy"],
+           "residual": recs[-1]["residual"], "converged": recs[-1]["converged"],
+           "fine_steps": recs[-1]["steps"]}
+    if keep:
+        return out, prev
+    prev.close()
+    return out
+
+
 def converge_network_start(pr, coarse, tol, gate, max_steps, check_every=1000, tuning=None,
                            infer_s=0.0):
     """The network start refined to the stop rule: relative energy change below tol for 8
@@ -326,12 +411,29 @@ def converge_main(a):
     """Both starts to the converged state at C5's size, in two phases that fit one GPU
     call each: `--phase network` trains the network and runs its start to the gated stop
     rule; `--phase tf --target E R` runs the TF start until its energy and GP residual are
-    at or below E and R (the network start's final state), or --max-steps."""
+    at or below E and R (the network start's final state), or --max-steps.
+    `--phase continuation` runs the coarse-to-fine solve (continuation_solve) from the
+    network's field and from the TF start."""
     test = np.array(a.test)
     pr = fine_problem(test, a.fine, a.fine_dtau)
     tuning = (2, 32, 1)
     log = lambda m: print(m, flush=True)  # noqa: E731
-    if a.phase == "network":
+    if a.phase == "continuation":
+        # the coarse-to-fine solve from the network's field and from the TF start
+        m, scale, info = train_network(samples=a.samples, train_n=a.train_n, epochs=a.epochs,
+                                       log=log)
+        try:
+            coarse, infer_s = predict(m, scale, test)
+        finally:
+            m.close()
+        net = continuation_solve(pr, coarse=coarse, tol=a.tol, gate=a.gate,
+                                 max_steps=a.max_steps, tuning=tuning, log=log)
+        net["wall_s"] += infer_s
+        tf = continuation_solve(pr, tol=a.tol, gate=a.gate, max_steps=a.max_steps,
+                                tuning=tuning, log=log)
+        out = {"phase": "continuation", "network": info, "inference_s": infer_s,
+               "network_start": net, "tf_start": tf}
+    elif a.phase == "network":
         m, scale, info = train_network(samples=a.samples, train_n=a.train_n, epochs=a.epochs,
                                        log=log)
         try:
@@ -375,8 +477,9 @@ def main():
                     help="also run both starts this many steps without the stop rule and "
                          "compare steps to equal energy along the way")
     ap.add_argument("--out", default="gpurun_out/warmstart")
-    ap.add_argument("--phase", choices=["network", "tf"], default=None,
-                    help="converged comparison in two phases (see converge_main)")
+    ap.add_argument("--phase", choices=["network", "tf", "continuation"], default=None,
+                    help="converged comparison in two phases, or the coarse-to-fine solve "
+                         "from both starts (see converge_main)")
     ap.add_argument("--gate", type=float, default=5e-4,
                     help="GP-residual gate of the stop rule for --phase network")
     ap.add_argument("--target", type=float, nargs=2, default=None,
