@@ -62,6 +62,14 @@ def test_bench_json_line_at_64():
     assert ws["network_start"]["steps"] == 2000
     assert ws["tf_cap_steps"] == 10000 and ws["tf_start"]["steps"] <= 10000
     assert ("speedup_steps" in ws) != ("speedup_steps_lower_bound" in ws)
+    cont = ws["continuation"]
+    for name in ("network_start", "tf_start"):
+        assert [lv["shape"][0] for lv in cont[name]["levels"]] == [21, 32, 64]
+        assert cont[name]["wall_s"] > 0
+    plain = cont["plain_tf_run"]
+    assert plain["reached_continuation_state"] == (
+        plain["energy"] <= cont["tf_start"]["energy"] and
+        plain["residual"] <= cont["tf_start"]["residual"])
 
 
 @pytest.mark.gpu
