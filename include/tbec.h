@@ -376,6 +376,11 @@ int tbec_upload_coarse(tbec_handle *h, const float *coarse, int32_t cn);
  * solves on one grid): the step between the levels of a coarse-to-fine ground-state
  * solve, for callers that relax a state on a coarser grid first. */
 int tbec_prolong(tbec_handle *dst, tbec_handle *src);
+/* tbec_prolong for every instance of a batch: instance i of dst (its next solve's start)
+ * from instance i of src; the batches hold as many instances, each pair on one box, in
+ * one precision.  Synchronises.  No reference counterpart; the coarse-to-fine solve of
+ * an ensemble. */
+int tbec_batch_prolong(tbec_batch *dst, tbec_batch *src);
 
 const char *tbec_last_error(void);
 /* Library version string and the gfx target it was built for. */

@@ -124,3 +124,38 @@ def test_continuation_reaches_the_one_grid_ground_state():
     assert d["residual"] <= gate and cont["residual"] <= gate
     assert abs(cont["energy"] - d["energy"]) <= 2e-7 * abs(d["energy"])
     assert np.linalg.norm(lev - plain) / np.linalg.norm(plain) < 2e-4
+
+
+def test_batch_prolong_equals_single_prolong():
+    """Every instance of tbec_batch_prolong equals tbec_prolong of the same state bit for
+    bit (grids and boxes differ per instance), and the result starts a batch solve."""
+    fine = [initial.Problem((20, 16, 24), (6.0, 5.0, 7.0), (1.0, 1.1, 0.9), 80.0, 0.3, -1e-3j),
+            initial.Problem((12, 18, 10), (4.0, 6.0, 5.0), (0.9, 0.9, 1.2), 40.0, 0.0, -2e-3j)]
+    coarse = [initial.Problem((9, 11, 7), p.box, p.omega, p.g, p.Omega, p.dt) for p in fine]
+    announce("batch prolong, 2 instances")
+    psis = [random_psi(p.shape, 7 + i) for i, p in enumerate(coarse)]
+    with tbec.Batch(coarse, prec=32) as bc, tbec.Batch(fine, prec=32) as bf:
+        for i, psi in enumerate(psis):
+            bc.upload(i, psi)
+        tbec.batch_prolong(bf, bc)
+        got = [bf.download(i) for i in range(2)]
+        res = bf.imag_solve(20, 1e-12, 1.0, 10)
+        assert all(r["steps"] == 20 for r in res)
+    for i, (pc, pf) in enumerate(zip(coarse, fine)):
+        with tbec.Condensate.from_problem(pc, prec=32) as s, \
+                tbec.Condensate.from_problem(pf, prec=32) as d:
+            s.upload(psis[i])
+            tbec.prolong(d, s)
+            assert np.array_equal(got[i], d.download()), i
+
+
+def test_batch_prolong_refuses_mismatched_batches():
+    announce("batch prolong argument checks")
+    p = initial.Problem((8, 8, 8), (4.0, 4.0, 4.0), (1.0, 1.0, 1.0), 10.0, 0.0, -1e-3j)
+    q = initial.Problem((8, 8, 8), (4.0, 4.0, 5.0), (1.0, 1.0, 1.0), 10.0, 0.0, -1e-3j)
+    with tbec.Batch([p, p], prec=32) as a, tbec.Batch([p], prec=32) as b, \
+            tbec.Batch([p, q], prec=32) as c, tbec.Batch([p, p], prec=64) as d:
+        for dst, src in ((b, a), (c, a), (d, a), (a, a)):
+            with pytest.raises(tbec.TbecError) as e:
+                tbec.batch_prolong(dst, src)
+            assert e.value.code == -1

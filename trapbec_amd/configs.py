@@ -7,7 +7,8 @@ c2: real time at 256^3 in FP64 and FP32 from the same state: psi relative L2 err
 c3: imaginary-time ground state in a rotating trap at 256^3, FP32 bulk with FP64
     reductions against the FP64 run: energy error, vortex-lattice count.
 c4: 48 condensates of uneven grids to energy convergence: one batched solve against
-    solving them one at a time on the same GPU.
+    solving them one at a time on the same GPU; --continuation solves each on its grid
+    halved first (coarse-to-fine).
 """
 from __future__ import annotations
 
@@ -191,6 +192,94 @@ def run_c4(n_inst=48, tol=1e-10, max_steps=20000, check_every=100, prec=32, gate
     return out
 
 
+def _midplane_cores(pr, psi, prec):
+    with tbec.Condensate.from_problem(pr, prec=prec) as c:
+        c.upload(psi)
+        v = c.vortex_cores(1e-5 * float(np.max(np.abs(psi)) ** 2), 0)
+    return int(np.sum(v[:, 2] == pr.shape[2] // 2)) if len(v) else 0
+
+
+def run_c4_continuation(n_inst=48, tol=1e-10, gate=5e-4, coarse_gate=1e-4, factor=2,
+                        max_steps=100000, coarse_max=400000, check_every=100, prec=32):
+    """C4 by the coarse-to-fine solve.  Each instance's start is interpolated onto its
+    grid divided by `factor` (dtau there from the stability bound), solved to the gated
+    stop rule with coarse_gate, interpolated back and solved to the gated rule with
+    `gate` at its own grid and dtau.  Once as two batches (tbec_batch_prolong between
+    them), once one instance at a time (tbec_prolong)."""
+    from trapbec_amd import autotune
+    from trapbec_amd.warmstart import RK3_REAL_LIMIT, max_rate
+    probs = initial.config_c4(n_inst=n_inst)
+    coarse = []
+    for p, _ in probs:
+        shape = tuple(max(2, n // factor) for n in p.shape)
+        dt = max(abs(complex(p.dt)), 0.95 * RK3_REAL_LIMIT / max_rate(p, shape))
+        coarse.append(initial.Problem(shape, p.box, p.omega, p.g, p.Omega, -dt * 1j, False,
+                                      dict(p.meta)))
+    out = {"config": "c4", "method": "coarse-to-fine", "instances": n_inst, "tol": tol,
+           "gate": gate, "coarse_gate": coarse_gate, "factor": factor,
+           "coarse_dtau": [abs(complex(c.dt)) for c in coarse],
+           "total_points": int(sum(np.prod(p.shape) for p, _ in probs))}
+    tb = autotune.tune_batch([p for p, _ in probs], prec=prec)
+    print(f"[c4] batch tuning: tile {tb['tile']} seg {tb['seg']}", flush=True)
+    t0 = time.time()
+    with tbec.Batch([p for p, _ in probs], prec=prec) as bf, \
+            tbec.Batch(coarse, prec=prec) as bc:
+        bf.set_tuning(tb["tile"], tb["seg"])
+        bc.set_tuning(tb["tile"], tb["seg"])
+        for i, (_, psi0) in enumerate(probs):
+            bf.upload(i, psi0)
+        t1 = time.time()
+        tbec.batch_prolong(bc, bf)
+        bc.set_residual_gate(coarse_gate)
+        rc = bc.imag_solve(coarse_max, tol, 1.0, check_every)
+        t2 = time.time()
+        print(f"[c4] coarse batch: {t2 - t1:.1f} s, "
+              f"{sum(r['converged'] for r in rc)}/{n_inst} converged", flush=True)
+        tbec.batch_prolong(bf, bc)
+        bf.set_residual_gate(gate)
+        rf = bf.imag_solve(max_steps, tol, 1.0, check_every)
+        t3 = time.time()
+        fin = [bf.download(i) for i in range(n_inst)]
+    out["batched"] = {"wall_s": t3 - t1, "coarse_s": t2 - t1, "fine_s": t3 - t2,
+                      "setup_s": t1 - t0,
+                      "converged": int(sum(r["converged"] for r in rf)),
+                      "coarse_converged": int(sum(r["converged"] for r in rc)),
+                      "coarse_steps": [int(r["steps"]) for r in rc],
+                      "steps": [int(r["steps"]) for r in rf],
+                      "energy": [r["energy"] for r in rf],
+                      "residual": [r["residual"] for r in rf],
+                      "midplane_cores": [_midplane_cores(p, psi, prec)
+                                         for (p, _), psi in zip(probs, fin)]}
+    del fin
+    print(f"[c4] batched: {t3 - t1:.1f} s, {out['batched']['converged']}/{n_inst} converged",
+          flush=True)
+    t_seq, seq = 0.0, []
+    for k, ((p, psi0), pc) in enumerate(zip(probs, coarse)):
+        with tbec.Condensate.from_problem(p, prec=prec) as cf, \
+                tbec.Condensate.from_problem(pc, prec=prec) as cc:
+            cf.upload(psi0)
+            cf.sync()
+            t1 = time.time()
+            tbec.prolong(cc, cf)
+            cc.set_residual_gate(coarse_gate)
+            r1 = cc.imag_solve(coarse_max, tol, 1.0, check_every)
+            tbec.prolong(cf, cc)
+            cf.set_residual_gate(gate)
+            r2 = cf.imag_solve(max_steps, tol, 1.0, check_every)
+            cf.sync()
+            t_seq += time.time() - t1
+            seq.append((r1, r2))
+        print(f"[c4] sequential {k + 1}/{n_inst} {p.shape}: {int(r1['steps'])} coarse + "
+              f"{int(r2['steps'])} fine steps, {t_seq:.1f} s so far", flush=True)
+    out["sequential"] = {"wall_s": t_seq,
+                         "converged": int(sum(r2["converged"] for _, r2 in seq)),
+                         "coarse_steps": [int(r1["steps"]) for r1, _ in seq],
+                         "steps": [int(r2["steps"]) for _, r2 in seq],
+                         "energy": [r2["energy"] for _, r2 in seq]}
+    out["speedup"] = t_seq / out["batched"]["wall_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("config", choices=["c2", "c3", "c4"])
@@ -199,15 +288,21 @@ def main():
     ap.add_argument("--max-steps", type=int, default=None)
     ap.add_argument("--gate", type=float, default=0.0,
                     help="c3, c4: GP-residual gate of the stop rule (0: none)")
+    ap.add_argument("--continuation", action="store_true",
+                    help="c4: the coarse-to-fine solve (grid / 2 first), gate 5e-4 unless "
+                         "--gate is given")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     if a.config == "c2":
         r = run_c2(n=a.n)
     elif a.config == "c3":
         r = run_c3(n=a.n, max_steps=a.max_steps or 200000, gate=a.gate)
+    elif a.continuation:
+        r = run_c4_continuation(max_steps=a.max_steps or 100000, gate=a.gate or 5e-4)
     else:
         r = run_c4(max_steps=a.max_steps or 20000, gate=a.gate)
-    with open(os.path.join(a.out, f"{a.config}.json"), "w") as f:
+    name = f"{a.config}_continuation" if a.continuation else a.config
+    with open(os.path.join(a.out, f"{name}.json"), "w") as f:
         json.dump(r, f, indent=1)
     print(json.dumps(r), flush=True)
 

@@ -431,6 +431,33 @@ int tbec_batch_download_psi(tbec_batch *b, int32_t i, void *psi_host) {
   return 0;
 }
 
+int tbec_batch_prolong(tbec_batch *dst, tbec_batch *src) {
+  if (!dst || !src || dst == src) return set_error(TBEC_ERR_ARG, "prolongation needs two batches");
+  if (dst->n != src->n) return set_error(TBEC_ERR_ARG, "batches of different instance counts");
+  if (dst->prec != src->prec)
+    return set_error(TBEC_ERR_ARG, "prolongation between batches of different precision");
+  for (int i = 0; i < dst->n; ++i)
+    if (!same_box(dst->p[i], src->p[i]))
+      return set_error(TBEC_ERR_ARG, "instance " + std::to_string(i) + ": different boxes");
+  // the source's pending steps finish before its states are read
+  BCHK(hipStreamSynchronize(src->stream));
+  const double one = 1.0;
+  for (int i = 0; i < dst->n; ++i) {
+    const Layout &Ls = src->L[i], &Ld = dst->L[i];
+    const GridView sv{src->p[i].nx, src->p[i].ny, src->p[i].nz, Ls.sy, Ls.sx, Ls.base};
+    const GridView dv{dst->p[i].nx, dst->p[i].ny, dst->p[i].nz, Ld.sy, Ld.sx, Ld.base};
+    BCHK(launch_prolong(dst->prec, src->bufs[2 * i + src->cur[i]], sv,
+                        src->d_state + (size_t)i * ST_COUNT + ST_SCALE,
+                        dst->bufs[2 * i + dst->parity], dv, dst->stream));
+    // the interpolated state carries its source's renormalisation: none is pending
+    BCHK(hipMemcpyAsync(dst->d_state + (size_t)i * ST_COUNT + ST_SCALE, &one, sizeof(double),
+                        hipMemcpyHostToDevice, dst->stream));
+  }
+  BCHK(hipStreamSynchronize(dst->stream));
+  for (int i = 0; i < dst->n; ++i) dst->cur[i] = dst->parity;
+  return 0;
+}
+
 int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double norm,
                           int check_every, tbec_solve_result *out) {
   if (!b || !out || max_steps < 1 || check_every < 1 || !(norm > 0))

@@ -133,6 +133,16 @@ long long diagnose_partial_slots(tbec_handle *h);
 // preparing the physics of a step at time t would allocate a device table
 bool physics_would_allocate(const tbec_handle *h, double t);
 int tbec_reserve_partials(tbec_handle *h, int nb);
+// Interior extent and padded strides of one grid in device memory.
+struct GridView {
+  int nx, ny, nz;
+  long long sy, sx, base;
+};
+// Trilinear interpolation of src (scaled by *scale) at dst's cell centres, same box
+// (tbec_prolong); enqueued on `stream`.
+hipError_t launch_prolong(int prec, const void *src, const GridView &s, const double *scale,
+                          void *dst, const GridView &d, hipStream_t stream);
+bool same_box(const tbec_params &a, const tbec_params &b);
 bool handle_rot(const tbec_handle *h);
 int set_error(int code, const std::string &msg);
 }  // namespace tbec

@@ -64,6 +64,30 @@ __global__ void k_prolong(const cplx<T> *__restrict__ src, int mx, int my, int m
 
 }  // namespace
 
+hipError_t tbec::launch_prolong(int prec, const void *src, const GridView &s,
+                                const double *scale, void *dst, const GridView &d,
+                                hipStream_t stream) {
+  const long long n = (long long)d.nx * d.ny * d.nz;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (prec == TBEC_F32)
+    hipLaunchKernelGGL(k_prolong<float>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const cplx<float> *)src, s.nx, s.ny, s.nz, s.sy, s.sx, s.base, scale,
+                       (cplx<float> *)dst, d.nx, d.ny, d.nz, d.sy, d.sx, d.base);
+  else
+    hipLaunchKernelGGL(k_prolong<double>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (const cplx<double> *)src, s.nx, s.ny, s.nz, s.sy, s.sx, s.base, scale,
+                       (cplx<double> *)dst, d.nx, d.ny, d.nz, d.sy, d.sx, d.base);
+  return hipGetLastError();
+}
+
+bool tbec::same_box(const tbec_params &a, const tbec_params &b) {
+  const double La[3] = {a.Lx, a.Ly, a.Lz}, Lb[3] = {b.Lx, b.Ly, b.Lz};
+  for (int d = 0; d < 3; ++d)
+    if (!(std::fabs(La[d] - Lb[d]) <= 1e-12 * std::fabs(La[d]))) return false;
+  return true;
+}
+
 extern "C" int tbec_prolong(tbec_handle *dst, tbec_handle *src) {
   if (!dst || !src) return set_error(TBEC_ERR_ARG, "null handle");
   if (dst == src) return set_error(TBEC_ERR_ARG, "prolongation needs two handles");
@@ -71,31 +95,15 @@ extern "C" int tbec_prolong(tbec_handle *dst, tbec_handle *src) {
     return set_error(TBEC_ERR_ARG, "prolongation between handles of different precision");
   if (dst->device != src->device)
     return set_error(TBEC_ERR_ARG, "prolongation between handles on different devices");
-  const double Ld[3] = {dst->p.Lx, dst->p.Ly, dst->p.Lz};
-  const double Ls[3] = {src->p.Lx, src->p.Ly, src->p.Lz};
-  for (int d = 0; d < 3; ++d)
-    if (!(std::fabs(Ld[d] - Ls[d]) <= 1e-12 * std::fabs(Ld[d])))
-      return set_error(TBEC_ERR_ARG, "prolongation between different boxes");
+  if (!same_box(dst->p, src->p))
+    return set_error(TBEC_ERR_ARG, "prolongation between different boxes");
   // the source's pending steps finish before its psi is read
   hipError_t e = hipStreamSynchronize(src->stream);
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, hipGetErrorString(e));
-  const long long n = (long long)dst->p.nx * dst->p.ny * dst->p.nz;
-  long long blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  const double *scale = src->dstate + ST_SCALE;
-  if (dst->p.prec == TBEC_F32)
-    hipLaunchKernelGGL(k_prolong<float>, dim3((unsigned)blocks), dim3(256), 0, dst->stream,
-                       (const cplx<float> *)src->psi[src->cur], src->p.nx, src->p.ny,
-                       src->p.nz, src->sy, src->sx, src->base, scale,
-                       (cplx<float> *)dst->psi[dst->cur], dst->p.nx, dst->p.ny, dst->p.nz,
-                       dst->sy, dst->sx, dst->base);
-  else
-    hipLaunchKernelGGL(k_prolong<double>, dim3((unsigned)blocks), dim3(256), 0, dst->stream,
-                       (const cplx<double> *)src->psi[src->cur], src->p.nx, src->p.ny,
-                       src->p.nz, src->sy, src->sx, src->base, scale,
-                       (cplx<double> *)dst->psi[dst->cur], dst->p.nx, dst->p.ny, dst->p.nz,
-                       dst->sy, dst->sx, dst->base);
-  e = hipGetLastError();
+  const GridView sv{src->p.nx, src->p.ny, src->p.nz, src->sy, src->sx, src->base};
+  const GridView dv{dst->p.nx, dst->p.ny, dst->p.nz, dst->sy, dst->sx, dst->base};
+  e = launch_prolong(dst->p.prec, src->psi[src->cur], sv, src->dstate + ST_SCALE,
+                     dst->psi[dst->cur], dv, dst->stream);
   if (e != hipSuccess) return set_error(TBEC_ERR_HIP, hipGetErrorString(e));
   const int rc = tbec_reset_state(dst);
   if (rc) return rc;

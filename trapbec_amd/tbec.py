@@ -101,7 +101,8 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
            "tbec_mlp_num_params", "tbec_mlp_get_params", "tbec_mlp_set_params",
            "tbec_mlp_forward", "tbec_mlp_loss_grad", "tbec_mlp_set_data", "tbec_mlp_train",
-           "tbec_upload_coarse", "tbec_prolong", "tbec_set_xcd_remap", "tbec_init_state",
+           "tbec_upload_coarse", "tbec_prolong", "tbec_batch_prolong",
+           "tbec_set_xcd_remap", "tbec_init_state",
            "tbec_set_w_storage", "tbec_velocity", "tbec_qpressure", "tbec_set_boundaries",
            "tbec_set_physics", "tbec_set_potential_table", "tbec_potential", "tbec_set_time",
            "tbec_imprint_vortex", "tbec_reference_state", "tbec_reference_insert"]
@@ -562,3 +563,11 @@ def prolong(dst: Condensate, src: Condensate):
     L = load()
     L.tbec_prolong.argtypes = [C.c_void_p, C.c_void_p]
     _check(L.tbec_prolong(dst._h, src._h))
+
+
+def batch_prolong(dst: "Batch", src: "Batch"):
+    """tbec_prolong for every instance: instance i of `dst` starts from instance i of
+    `src` interpolated onto its grid; synchronises."""
+    L = load()
+    L.tbec_batch_prolong.argtypes = [C.c_void_p, C.c_void_p]
+    _check(L.tbec_batch_prolong(dst._h, src._h))
