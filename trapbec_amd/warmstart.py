@@ -315,14 +315,16 @@ RK3_REAL_LIMIT = 2.5127   # |z| where 1 + z + z^2/2 + z^3/6 leaves the unit disc
 def max_rate(pr, shape=None) -> float:
     """Upper bound of the imaginary-time operator's largest eigenvalue on `shape` (default
     pr.shape): the 6th-order -1/2 Laplacian at the grid's highest wavenumber
-    (3.0222/h^2 per axis), the trap at the box corner, the frame term's 6th-order first
-    derivative (1.8333/h) times |Omega| and the box's largest radius, and 2 mu_TF for the
-    nonlinearity (mu_TF of the norm-1 Thomas-Fermi profile)."""
+    (1088/360 / h^2 per axis: the stencil's symbol at theta = pi), the trap at the box
+    corner, |Omega| times the box's largest radius times the 6th-order first derivative's
+    bound (110/60 / h, its summed |coefficients|), and 2 mu_TF for the nonlinearity
+    (mu_TF of the norm-1 Thomas-Fermi profile)."""
     shape = shape or pr.shape
     h = [L / n for L, n in zip(pr.box, shape)]
-    kin = sum(3.0222 / hh ** 2 for hh in h)
+    kin = sum(1088.0 / 360.0 / hh ** 2 for hh in h)
     v = 0.5 * sum((w * L / 2) ** 2 for w, L in zip(pr.omega, pr.box))
-    rot = abs(pr.Omega) * math.hypot(pr.box[0] / 2, pr.box[1] / 2) * 1.8333 / min(h[0], h[1])
+    rot = abs(pr.Omega) * math.hypot(pr.box[0] / 2, pr.box[1] / 2) * (110.0 / 60.0) / \
+        min(h[0], h[1])
     wbar3 = pr.omega[0] * pr.omega[1] * pr.omega[2]
     mu = 0.5 * (15.0 * pr.g * wbar3 / (4.0 * math.pi)) ** 0.4
     return kin + v + rot + 2.0 * mu
