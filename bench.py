@@ -349,14 +349,14 @@ def warm_start_leg(args):
     dtau = 8e-4
     pr = ws.fine_problem(WARM_TEST, args.warm_fine, dtau)
     r = ws.equal_state_comparison(pr, coarse, args.warm_steps, args.warm_cap,
-                                  tuning=(2, 32, 1), infer_s=infer_s, log=log)
+                                  tuning=ws.C5_TUNING, infer_s=infer_s, log=log)
     r.pop("records")
     # the coarse-to-fine solve (128^3 -> 192^3 -> fine, gated stop rule 1e-10 with residual
     # <= 5e-4 at the fine grid) from the network's field and from the TF start; the plain
     # TF run above bounds how long one-grid imaginary time takes to reach that state
     cont = {}
     for name, field in (("network_start", coarse), ("tf_start", None)):
-        cont[name] = ws.continuation_solve(pr, coarse=field, tuning=(2, 32, 1), log=log)
+        cont[name] = ws.continuation_solve(pr, coarse=field, tuning=ws.C5_TUNING, log=log)
         cont[name].pop("fine_steps")
     cont["network_start"]["wall_s"] += infer_s
     plain = r["tf_start"]

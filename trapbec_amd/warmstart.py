@@ -36,6 +36,10 @@ RANGES = np.array([[0.8, 1.2],      # w_perp
                    [0.0, 0.8],      # Omega
                    [math.log(0.5), math.log(2.0)]])  # log N
 COARSE = 32
+# (tile, segment, XCD order) of the 384^3 rotating imaginary-time refinement: the fastest
+# of autotune.tune(problem=...) for C5's problem on MI355X (278.9 us per stage; 0.829 ms
+# per imaginary-time step against 0.902 with (2, 32, 1))
+C5_TUNING = (2, 128, 2)
 
 
 def sobol_params(n: int, seed: int) -> np.ndarray:
@@ -418,7 +422,7 @@ def converge_main(a):
     network's field and from the TF start."""
     test = np.array(a.test)
     pr = fine_problem(test, a.fine, a.fine_dtau)
-    tuning = (2, 32, 1)
+    tuning = C5_TUNING
     log = lambda m: print(m, flush=True)  # noqa: E731
     if a.phase == "continuation":
         # the coarse-to-fine solve from the network's field and from the TF start
