@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--imag-probe", action="store_true",
                     help="imaginary-time solve steps of the rotating 256^3 ground-state "
                          "problem (FP32 and FP64), for kernel traces")
+    ap.add_argument("--continuation-probe", action="store_true",
+                    help="the coarse-to-fine ground-state solve of the C5 problem from the "
+                         "TF start (128^3, 192^3, 384^3), for kernel traces")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the ground-state leg (network start vs TF start)")
     ap.add_argument("--warm-fine", type=int, default=384,
@@ -449,6 +452,18 @@ def pmc_probe(args):
                       "stage_bytes_moved_avg": n ** 3 * 64 / 3}), flush=True)
 
 
+def continuation_probe(args):
+    """The coarse-to-fine ground-state solve of C5's problem (384^3 rotating, dtau 8e-4)
+    from the TF start, for kernel traces: levels 128^3 and 192^3, then 384^3, each to the
+    gated stop rule.  Prints the per-level steps and wall times."""
+    import numpy as np
+    from trapbec_amd import warmstart as ws
+    pr = ws.fine_problem(np.asarray(WARM_TEST), 384, 8e-4)
+    r = ws.continuation_solve(pr, tuning=ws.C5_TUNING,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps({"continuation_probe": "C5 problem, TF start", **r}), flush=True)
+
+
 def imag_probe(args):
     """Imaginary-time solve steps of the rotating 256^3 ground-state problem
     (initial.config_c3: Omega = 0.7, dtau = 1e-3), FP32 then FP64, each with the tile
@@ -584,6 +599,9 @@ def main():
         return
     if args.imag_probe:
         imag_probe(args)
+        return
+    if args.continuation_probe:
+        continuation_probe(args)
         return
     pr, spec = initial.ic_spec("m", n)
     res = {}
