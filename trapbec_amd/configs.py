@@ -200,15 +200,26 @@ def _midplane_cores(pr, psi, prec):
 
 
 def run_c4_continuation(n_inst=48, tol=1e-10, gate=5e-4, coarse_gate=1e-4, factor=2,
-                        max_steps=100000, coarse_max=400000, check_every=100, prec=32):
+                        max_steps=100000, coarse_max=400000, check_every=100, prec=32,
+                        sequential=True, fine_bound=False, save=None):
     """C4 by the coarse-to-fine solve.  Each instance's start is interpolated onto its
     grid divided by `factor` (dtau there from the stability bound), solved to the gated
     stop rule with coarse_gate, interpolated back and solved to the gated rule with
-    `gate` at its own grid and dtau.  Once as two batches (tbec_batch_prolong between
-    them), once one instance at a time (tbec_prolong)."""
+    `gate` at its own grid, with its own dtau (1e-3) or, with fine_bound, the stability
+    bound's there too.  Once as two batches (tbec_batch_prolong between them), then
+    (sequential=True) once one instance at a time (tbec_prolong).  save(out), when given,
+    is called with the results so far after the batched solve."""
     from trapbec_amd import autotune
     from trapbec_amd.warmstart import RK3_REAL_LIMIT, max_rate
     probs = initial.config_c4(n_inst=n_inst)
+    if fine_bound:
+        # the configuration fixes the grid (dx = 0.2), not dtau: take the largest step
+        # the RK3 stability bound allows on that grid (the converged state does not
+        # depend on it; the per-step stop rule is stricter per unit of imaginary time)
+        probs = [(initial.Problem(p.shape, p.box, p.omega, p.g, p.Omega,
+                                  -max(abs(complex(p.dt)),
+                                       0.95 * RK3_REAL_LIMIT / max_rate(p)) * 1j,
+                                  False, dict(p.meta)), psi0) for p, psi0 in probs]
     coarse = []
     for p, _ in probs:
         shape = tuple(max(2, n // factor) for n in p.shape)
@@ -217,7 +228,9 @@ def run_c4_continuation(n_inst=48, tol=1e-10, gate=5e-4, coarse_gate=1e-4, facto
                                       dict(p.meta)))
     out = {"config": "c4", "method": "coarse-to-fine", "instances": n_inst, "tol": tol,
            "gate": gate, "coarse_gate": coarse_gate, "factor": factor,
+           "max_steps": max_steps, "coarse_max": coarse_max,
            "coarse_dtau": [abs(complex(c.dt)) for c in coarse],
+           "fine_dtau": [abs(complex(p.dt)) for p, _ in probs],
            "total_points": int(sum(np.prod(p.shape) for p, _ in probs))}
     tb = autotune.tune_batch([p for p, _ in probs], prec=prec)
     print(f"[c4] batch tuning: tile {tb['tile']} seg {tb['seg']}", flush=True)
@@ -253,6 +266,10 @@ def run_c4_continuation(n_inst=48, tol=1e-10, gate=5e-4, coarse_gate=1e-4, facto
     del fin
     print(f"[c4] batched: {t3 - t1:.1f} s, {out['batched']['converged']}/{n_inst} converged",
           flush=True)
+    if save:
+        save(out)
+    if not sequential:
+        return out
     t_seq, seq = 0.0, []
     for k, ((p, psi0), pc) in enumerate(zip(probs, coarse)):
         with tbec.Condensate.from_problem(p, prec=prec) as cf, \
@@ -291,19 +308,32 @@ def main():
     ap.add_argument("--continuation", action="store_true",
                     help="c4: the coarse-to-fine solve (grid / 2 first), gate 5e-4 unless "
                          "--gate is given")
+    ap.add_argument("--coarse-max", type=int, default=400000,
+                    help="c4 --continuation: step cap of the coarse level")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="c4 --continuation: skip the one-at-a-time comparison")
+    ap.add_argument("--fine-bound", action="store_true",
+                    help="c4 --continuation: dtau of each instance's own grid from the RK3 "
+                         "stability bound (default: 1e-3)")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
+    name = f"{a.config}_continuation" if a.continuation else a.config
+
+    def save(r):
+        with open(os.path.join(a.out, f"{name}.json"), "w") as f:
+            json.dump(r, f, indent=1)
+
     if a.config == "c2":
         r = run_c2(n=a.n)
     elif a.config == "c3":
         r = run_c3(n=a.n, max_steps=a.max_steps or 200000, gate=a.gate)
     elif a.continuation:
-        r = run_c4_continuation(max_steps=a.max_steps or 100000, gate=a.gate or 5e-4)
+        r = run_c4_continuation(max_steps=a.max_steps or 100000, gate=a.gate or 5e-4,
+                                coarse_max=a.coarse_max, sequential=not a.batch_only,
+                                fine_bound=a.fine_bound, save=save)
     else:
         r = run_c4(max_steps=a.max_steps or 20000, gate=a.gate)
-    name = f"{a.config}_continuation" if a.continuation else a.config
-    with open(os.path.join(a.out, f"{name}.json"), "w") as f:
-        json.dump(r, f, indent=1)
+    save(r)
     print(json.dumps(r), flush=True)
 
 
