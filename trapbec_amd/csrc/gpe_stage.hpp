@@ -762,7 +762,8 @@ struct BatchInst {
 struct BatchLaunch {
   const BatchInst *inst;
   const int *item_inst; // work item -> instance
-  int total_items;
+  const int *items;     // workgroup -> work item: the items of the unfinished instances
+  int total_items;      // workgroups in the launch (entries of items)
   int parity;           // psi[parity] is read, psi[parity ^ 1] written
   int stage;            // StageKind
   double A, B;
@@ -777,9 +778,11 @@ __global__ void __launch_bounds__((TZ / VZ) * TY, (stage_min_waves<T, (TZ / VZ) 
 gpe_stage_batch(const BatchLaunch L) {
   constexpr int NT = (TZ / VZ) * TY;
   // one work item per workgroup (the grid is total_items): no loop state stays live
-  // across the plane march
-  const int item = blockIdx.x;
-  if (item >= L.total_items) return;
+  // across the plane march.  The item list holds the unfinished instances' items only
+  // (the host compacts it as instances finish), so a tail of a few slow instances does
+  // not launch the finished ones' workgroups; partials stay at the item's own index.
+  if ((int)blockIdx.x >= L.total_items) return;
+  const int item = L.items[blockIdx.x];
   {
     const int i = L.item_inst[item];
     if (L.flags[4 * i] != 0) return;            // converged or failed: skip (uniform)

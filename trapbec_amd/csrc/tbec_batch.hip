@@ -37,10 +37,13 @@ struct tbec_batch {
   std::vector<int> cur;              // per-instance current buffer after a solve
   int parity = 0;
   int total_items = 0;
+  int active_items = 0;              // entries of d_items: the unfinished instances' items
+  int active_done = 0;               // finished instances when d_items was last built
   double rgate = 0.0;                // GP-residual gate of the stop rule (0: none)
   BatchInst *d_inst = nullptr;
   int *d_item_inst = nullptr;
   int *d_item_begin = nullptr;
+  int *d_items = nullptr;
   double *d_state = nullptr, *h_state = nullptr;
   int *d_flags = nullptr, *h_flags = nullptr;
   double *d_part = nullptr;
@@ -161,12 +164,14 @@ int build_table(tbec_batch *b) {
   if (b->d_inst) (void)hipFree(b->d_inst);
   if (b->d_item_inst) (void)hipFree(b->d_item_inst);
   if (b->d_item_begin) (void)hipFree(b->d_item_begin);
+  if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_part) (void)hipFree(b->d_part);
   b->d_inst = nullptr;
-  b->d_item_inst = b->d_item_begin = nullptr;
+  b->d_item_inst = b->d_item_begin = b->d_items = nullptr;
   b->d_part = nullptr;
   BCHK(hipMalloc(&b->d_inst, sizeof(BatchInst) * b->n));
   BCHK(hipMalloc(&b->d_item_inst, sizeof(int) * b->total_items));
+  BCHK(hipMalloc(&b->d_items, sizeof(int) * b->total_items));
   BCHK(hipMalloc(&b->d_item_begin, sizeof(int) * (b->n + 1)));
   BCHK(hipMalloc(&b->d_part, sizeof(double) * 4 * (size_t)b->total_items));
   BCHK(hipMemcpy(b->d_inst, inst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice));
@@ -174,6 +179,11 @@ int build_table(tbec_batch *b) {
                  hipMemcpyHostToDevice));
   BCHK(hipMemcpy(b->d_item_begin, b->item_begin.data(), sizeof(int) * (b->n + 1),
                  hipMemcpyHostToDevice));
+  std::vector<int> all(b->total_items);
+  for (int k = 0; k < b->total_items; ++k) all[k] = k;
+  BCHK(hipMemcpy(b->d_items, all.data(), sizeof(int) * b->total_items, hipMemcpyHostToDevice));
+  b->active_items = b->total_items;
+  b->active_done = 0;
   return 0;
 }
 
@@ -187,11 +197,39 @@ void drop_graphs(tbec_batch *b) {
   b->graph_steps = 0;
 }
 
+// Rebuild the launch's item list from the instances whose flag is 0 (every instance when
+// `flags` is null).  The captured graphs hold the grid size, so they are dropped; the
+// stream is drained before the list is rewritten.  Partials keep their item indices, so
+// every instance's arithmetic and reduction order are unchanged.
+int set_active_items(tbec_batch *b, const int *flags) {
+  std::vector<int> items;
+  items.reserve(b->total_items);
+  int done = 0;
+  for (int i = 0; i < b->n; ++i) {
+    if (flags && flags[4 * i] != 0) {
+      ++done;
+      continue;
+    }
+    for (int k = b->item_begin[i]; k < b->item_begin[i + 1]; ++k) items.push_back(k);
+  }
+  if (items.empty()) return 0;            // nothing left to launch: keep the last list
+  if ((int)items.size() == b->active_items && done == b->active_done) return 0;
+  drop_graphs(b);
+  BCHK(hipStreamSynchronize(b->stream));
+  BCHK(hipMemcpyAsync(b->d_items, items.data(), sizeof(int) * items.size(),
+                      hipMemcpyHostToDevice, b->stream));
+  BCHK(hipStreamSynchronize(b->stream));
+  b->active_items = (int)items.size();
+  b->active_done = done;
+  return 0;
+}
+
 int enqueue_batch_stage(tbec_batch *b, int kind) {
   BatchLaunch L;
   L.inst = b->d_inst;
   L.item_inst = b->d_item_inst;
-  L.total_items = b->total_items;
+  L.items = b->d_items;
+  L.total_items = b->active_items;
   L.parity = b->parity;
   L.stage = kind;
   L.A = kind == ST_RK1 ? 0.0 : (kind == ST_RK2 ? -2.0 / 3.0 : -1.0);
@@ -200,7 +238,7 @@ int enqueue_batch_stage(tbec_batch *b, int kind) {
   L.state = b->d_state;
   L.state_stride = ST_COUNT;
   L.flags = b->d_flags;
-  const int grid = b->total_items;   // one workgroup per (instance, tile) work item
+  const int grid = b->active_items;  // one workgroup per (instance, tile) work item
   hipError_t e = launch_batch(b->prec, b->tile, L, b->sum, b->rot, grid, b->stream);
   if (e != hipSuccess)
     return set_error(TBEC_ERR_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
@@ -353,6 +391,7 @@ int tbec_batch_destroy(tbec_batch *b) {
   if (b->d_inst) (void)hipFree(b->d_inst);
   if (b->d_item_inst) (void)hipFree(b->d_item_inst);
   if (b->d_item_begin) (void)hipFree(b->d_item_begin);
+  if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_part) (void)hipFree(b->d_part);
   if (b->d_state) (void)hipFree(b->d_state);
   if (b->d_flags) (void)hipFree(b->d_flags);
@@ -467,6 +506,7 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
     if (b->cur[i] != b->parity)
       return set_error(TBEC_ERR_STATE, "upload every instance before a batch solve");
   int rc = reset_states(b, norm, tol);
+  if (!rc) rc = set_active_items(b, nullptr);   // every instance starts unfinished
   if (rc) return rc;
   const int p0 = b->parity;
   long long chunk = check_every + (check_every & 1);
@@ -495,6 +535,11 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
                      b->n);
         last = now;
       }
+    }
+    // finished instances leave the launches (their workgroups would only exit)
+    if (!all && nconv > b->active_done) {
+      rc = set_active_items(b, b->h_flags);
+      if (rc) return rc;
     }
   }
   BCHK(hipMemcpyAsync(b->h_state, b->d_state, sizeof(double) * ST_COUNT * b->n,
