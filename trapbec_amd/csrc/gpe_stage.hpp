@@ -448,10 +448,12 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // count.  Each iteration then issues the same number of them on every path, and the
   // compiler's in-order wait counts let the loads of the next planes stay in flight
   // across the barrier; with a branch around a load or store, the wait before the
-  // barrier also waits for the prefetch issued just before it.  Measured at 512^3 on
-  // MI355X (A/B on one box): FP32 with w rebuilt -0.9% per step; the stored-w scheme
-  // (+4%) and FP64 (+3%) are slower with it, so they keep the branches.
-  constexpr bool UNI = TBEC_UNIFORM_VMEM && sizeof(T) == 4 && FIX && !STO;
+  // barrier also waits for the prefetch issued just before it.  It pays in the first RK
+  // stage, the one with the most arithmetic per byte: 512^3 FP32 stage 1 472 -> 437 us,
+  // rotating 525 -> 469 us (A/B on one box, kernel traces).  Stages 2 and 3, at the HBM
+  // rate for their mix, are 1.8% slower with it (the extra address work of the dummy
+  // halo loads), and the stored-w scheme and FP64 slower still, so they keep the branches.
+  constexpr bool UNI = TBEC_UNIFORM_VMEM && sizeof(T) == 4 && FIX && !STO && KIND == ST_RK1;
   const uint32_t ownb = own * (uint32_t)sizeof(C);
   const uint32_t hoffb = (UNI && !hl) ? kOffNone : hoff * (uint32_t)sizeof(C);
   auto scale_in = [&](C *o) {
