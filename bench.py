@@ -136,11 +136,13 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False, table=Fa
     c = tbec.Condensate.from_problem(pr, prec=prec, sum_mode=sm)
     c.set_w_storage(w_stored)
     tile, seg, remap = autotune.lookup(pr.shape, prec, sm, w_stored, pr.Omega != 0.0)
+    tile1, seg1 = autotune.lookup_first_stage(pr.shape, prec, sm, w_stored, pr.Omega != 0.0)
     if args.tile >= 0:
-        tile = args.tile
+        tile, tile1, seg1 = args.tile, -1, 0
     if args.seg > 0:
         seg = args.seg
     c.set_tuning(tile, seg)
+    c.set_first_stage_tuning(tile1, seg1)
     c.set_xcd_remap(remap)
     c.init_state(spec)            # seeded state generated in HBM (not timed)
     if table:
@@ -160,7 +162,8 @@ def run_gpu(args, prec, steps, spec, pr, sum_name=None, w_stored=False, table=Fa
     c.close()
     return {"ms_per_step": ms, "gpt_steps_per_s": gpts, "wall_s": wall,
             "achieved_Bps": achieved, "moved_Bps": moved, "tile": tile, "seg": seg,
-            "remap": remap, "ms_per_launch": ms / 3.0, "w_stored": w_stored}
+            "remap": remap, "first_stage": [tile1, seg1], "ms_per_launch": ms / 3.0,
+            "w_stored": w_stored}
 
 
 def stream_ceiling(n, prec, reps=20):
@@ -441,6 +444,7 @@ def pmc_probe(args):
     c = tbec.Condensate.from_problem(pr, prec=32, sum_mode=SUMS[args.fp32_sum])
     tile, seg, remap = autotune.lookup(pr.shape, 32, SUMS[args.fp32_sum])
     c.set_tuning(tile, seg)
+    c.set_first_stage_tuning(*autotune.lookup_first_stage(pr.shape, 32, SUMS[args.fp32_sum]))
     c.set_xcd_remap(remap)
     psi = torch.full((n ** 3,), 0.1 + 0.05j, dtype=torch.complex64, device="cuda:0")
     c.upload_device(psi.data_ptr())
@@ -661,7 +665,8 @@ def main():
                                f"zero ghosts, Omega=0",
                    "grid": [n, n, n], "stencil": "6th-order FD, fused RK3 stage",
                    "fp32_sum": args.fp32_sum, "w": "rebuilt", "tile": r["tile"],
-                   "seg": r["seg"], "xcd_remap": r["remap"]},
+                   "seg": r["seg"], "xcd_remap": r["remap"],
+                   "first_stage_tile_seg": r["first_stage"]},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
@@ -680,7 +685,8 @@ def main():
                 "frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
                 "frac_of_bytes_moved": round(v["moved_Bps"] / HBM_PEAK, 4),
                 "achieved_GBps": round(v["achieved_Bps"] / 1e9, 1),
-                "tile": v["tile"], "seg": v["seg"], "steps": steps}
+                "tile": v["tile"], "seg": v["seg"], "first_stage_tile_seg": v["first_stage"],
+                "steps": steps}
     sc = res["stream"][head]
     out["stream_ceiling"] = dict(sc, frac=round(out["value"] / sc["value"], 4),
                                  basis="copy (stage 1) + 2 x add (stages 2, 3) over "

@@ -234,6 +234,13 @@ int tbec_set_norm(tbec_handle *h, double norm);
  * segment (256 down to 8 planes) that still gives >= 1024 workgroups, which is also
  * the default at creation.  Drops captured graphs. */
 int tbec_set_tuning(tbec_handle *h, int tile, int seg);
+/* A tile shape and segment of its own for the first RK stage of real-time steps
+ * (TBEC_REAL_RK3), the stage that reads one array: tile = -1 (the default) runs it with
+ * the tbec_set_tuning choice.  seg = 0: automatic, as for tbec_set_tuning.  Every other
+ * stage, imaginary time and the diagnostics keep the handle's tuning.  No reference
+ * counterpart (a knob of the stage kernel, chosen by the autotuner).  Drops captured
+ * graphs. */
+int tbec_set_first_stage_tuning(tbec_handle *h, int tile, int seg);
 /* Boundary faces, as ghost_comm fills them (src/ghost.f90:27-192; `boundaries` and
  * `periodicz` of src/cparam.local): TBEC_BC_ZERO (default: ghosts 0), TBEC_BC_SOLID
  * (mirror about the edge point: ghost 0 = psi 2, -1 = 3, -2 = 4), TBEC_BC_PERIODIC

@@ -44,3 +44,52 @@ def test_tune_batch_picks_the_fastest_candidate():
     assert len(e["all"]) == tbec.NUM_TILES * len(autotune.SEGS)
     assert [e["tile"], e["seg"]] == e["all"][0][:2]
     assert round(e["ms_per_step"], 4) == min(a[2] for a in e["all"])
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_first_stage_tiling_changes_speed_not_results(prec):
+    """The first RK stage run with each tile shape of its own (the other stages with the
+    handle's tiling) gives the psi of the single tiling, on a ragged rotating grid."""
+    shape = (40, 36, 52)
+    pr = initial.Problem(shape, (8.0, 7.0, 9.0), (1.0, 1.1, 0.9), 50.0, 0.4, 1e-3)
+    psi0 = initial.tf_profile(shape, pr.box, pr.omega, pr.g, norm=1.0)[0] * \
+        initial.vortex_line_factor(shape, pr.box, 0.3, 0.1, 1, 0.4)
+    print(f"\n[case] first-stage tiling {shape} fp{prec}", flush=True)
+
+    def run(tile1, seg1):
+        with tbec.Condensate.from_problem(pr, prec=prec) as c:
+            c.set_tuning(2, 0)
+            c.set_first_stage_tuning(tile1, seg1)
+            c.upload(psi0)
+            c.step(3)           # eager
+            c.step(32)          # graph replays
+            return c.download()
+
+    ref = run(-1, 0)
+    tol = 2e-6 if prec == 32 else 1e-12
+    for tile1 in range(tbec.NUM_TILES):
+        for seg1 in (0, 16):
+            got = run(tile1, seg1)
+            err = float(np.abs(got - ref).max() / np.abs(ref).max())
+            print(f"first stage tile {tile1} seg {seg1}: rel max {err:.2e}", flush=True)
+            assert err < tol, (tile1, seg1, err)
+
+
+def test_first_stage_tuning_rejects_bad_arguments():
+    pr = initial.Problem((16, 16, 16), (8.0, 8.0, 8.0), (1.0, 1.0, 1.0), 50.0, 0.0, 1e-3)
+    with tbec.Condensate.from_problem(pr, prec=32) as c:
+        for tile, seg in ((tbec.NUM_TILES, 0), (-2, 0), (0, -1), (0, 5000)):
+            with pytest.raises(tbec.TbecError):
+                c.set_first_stage_tuning(tile, seg)
+        c.set_first_stage_tuning(-1, 0)
+
+
+def test_tune_first_stage_records_the_fastest(tmp_path):
+    shape, prec = (40, 36, 48), 32
+    table = str(tmp_path / "tuning.json")
+    print(f"\n[case] first-stage autotune {shape} fp{prec}", flush=True)
+    rec = autotune.tune_first_stage(shape, prec, nstages=6, table=table, reps=1)
+    assert len(rec["all"]) == 1 + tbec.NUM_TILES * len(autotune.SEGS)
+    assert [rec["tile"], rec["seg"]] == rec["all"][0][:2]
+    assert round(rec["us_per_stage"], 2) == min(a[2] for a in rec["all"])
+    assert autotune.lookup_first_stage(shape, prec, table=table) == (rec["tile"], rec["seg"])

@@ -366,8 +366,8 @@ static Layout layout_of(const tbec_handle *h) {
 }
 
 static int fill_args(tbec_handle *h, StageArgs &a, int stage, int src, double A, double B,
-                     double cre, double cim) {
-  fill_static_args(h->p, layout_of(h), h->tile, h->seg, a);
+                     double cre, double cim, int tile, int seg) {
+  fill_static_args(h->p, layout_of(h), tile, seg, a);
   a.psi_in = h->psi[src];
   a.psi_out = h->psi[src ^ 1];
   a.w = h->w;
@@ -409,7 +409,11 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce, double *part
   else if (kind == ST_RK2) { A = -2.0 / 3.0; B = 1.0; }
   else if (kind == ST_RK3) { A = -1.0; B = 0.5; }
   else if (kind == ST_EULER) { cre = 0.5 * h->p.dt_re; cim = 0.5 * h->p.dt_im; }  // dt*0.5
-  int nb = fill_args(h, a, kind, h->cur, A, B, cre, cim);
+  // the first RK stage of a real-time step may run with a tiling of its own
+  // (tbec_set_first_stage_tuning); every other stage uses the handle's
+  const bool own1 = h->tile1 >= 0 && kind == ST_RK1 && !imag && !reduce;
+  const int tile = own1 ? h->tile1 : h->tile, seg = own1 ? h->seg1 : h->seg;
+  int nb = fill_args(h, a, kind, h->cur, A, B, cre, cim, tile, seg);
   const bool want_red = imag || reduce;
   if (want_red) {
     if (!part) {
@@ -448,7 +452,7 @@ int enqueue_stage(tbec_handle *h, int kind, bool imag, bool reduce, double *part
   }
   int rcg = refresh_ghosts(h);
   if (rcg) return rcg;
-  hipError_t e = launch_stage(h->p.prec, h->tile, a, h->p.sum_mode, h->ph_mode, want_red,
+  hipError_t e = launch_stage(h->p.prec, tile, a, h->p.sum_mode, h->ph_mode, want_red,
                               h->stream);
   if (e != hipSuccess)
     return fail(TBEC_ERR_HIP, "stage launch failed: %s", hipGetErrorString(e));
@@ -682,6 +686,19 @@ int tbec_set_tuning(tbec_handle *h, int tile, int seg) {
   return 0;
 }
 
+int tbec_set_first_stage_tuning(tbec_handle *h, int tile, int seg) {
+  if (!h) return fail(TBEC_ERR_ARG, "null handle");
+  if (tile < -1 || tile >= kNumTiles) return fail(TBEC_ERR_ARG, "tile id %d out of range", tile);
+  if (seg < 0 || seg > 4096) return fail(TBEC_ERR_ARG, "segment length %d out of range", seg);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  tbec_drop_graphs(h);
+  h->tile1 = tile;
+  h->seg1 = tile < 0 ? 0
+                     : (seg > 0 ? seg : auto_seg(tiles_per_plane(h->p, tile), h->p.nx,
+                                                  kAutoSegBlocks));
+  return 0;
+}
+
 int tbec_set_boundaries(tbec_handle *h, int boundaries, int periodicz) {
   if (!h) return fail(TBEC_ERR_ARG, "null handle");
   if (boundaries < TBEC_BC_ZERO || boundaries > TBEC_BC_PERIODIC)
@@ -728,7 +745,7 @@ void *tbec_stream(tbec_handle *h) { return h ? (void *)h->stream : nullptr; }
 // ------------------------------------------------------ state helpers -----------
 int fill_args_count(tbec_handle *h) {
   StageArgs a;
-  return fill_args(h, a, ST_RHS, h->cur, 0, 0, 0, 0);
+  return fill_args(h, a, ST_RHS, h->cur, 0, 0, 0, 0, h->tile, h->seg);
 }
 
 namespace tbec {

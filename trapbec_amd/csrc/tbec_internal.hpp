@@ -81,6 +81,7 @@ struct tbec_handle {
   tbec_params p{};
   int device = 0;                       // HIP device current at tbec_create
   int tile = 0, seg = 64;
+  int tile1 = -1, seg1 = 0;             // first RK stage in real time (tile1 < 0: as tile)
   int remap = 1;                        // XCD-aware tile order
   int w_stored = 0;                     // 1: RK3 stores w (10 transfers), 0: rebuilt (8)
   int bc = 0, periodicz = 0;            // boundary faces (TBEC_BC_*), z periodic

@@ -95,7 +95,8 @@ EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi"
            "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve",
            "tbec_set_residual_gate", "tbec_batch_set_residual_gate", "tbec_set_norm",
            "tbec_sync", "tbec_stream", "tbec_stream_rates",
-           "tbec_time_stages", "tbec_set_tuning", "tbec_last_error", "tbec_version",
+           "tbec_time_stages", "tbec_set_tuning", "tbec_set_first_stage_tuning",
+           "tbec_last_error", "tbec_version",
            "tbec_batch_create", "tbec_batch_destroy", "tbec_batch_upload_psi",
            "tbec_batch_download_psi", "tbec_batch_imag_solve", "tbec_batch_set_tuning",
            "tbec_batch_stream", "tbec_vortex_cores", "tbec_mlp_create", "tbec_mlp_destroy",
@@ -136,6 +137,7 @@ def load(path: str = LIB_PATH):
     L.tbec_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_batch_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
+    L.tbec_set_first_stage_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
     L.tbec_set_w_storage.argtypes = [H, C.c_int]
     L.tbec_set_boundaries.argtypes = [H, C.c_int, C.c_int]
@@ -312,6 +314,11 @@ class Condensate:
 
     def set_tuning(self, tile: int, seg: int):
         _check(load().tbec_set_tuning(self._h, tile, seg))
+
+    def set_first_stage_tuning(self, tile: int, seg: int = 0):
+        """Tile and segment of the first RK stage of real-time steps (tile -1: as
+        set_tuning)."""
+        _check(load().tbec_set_first_stage_tuning(self._h, tile, seg))
 
     def set_xcd_remap(self, on: int):
         """0 dispatch order, 1 XCD-contiguous tile order for every stage, 2 for the
