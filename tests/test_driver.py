@@ -74,6 +74,21 @@ def test_driver_initial_state_matches_python(tmp_path):
     assert np.abs(psi - ref).max() < 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("dt, warned", [(0.01, False), (0.2, True)])
+def test_driver_timestep_warning(tmp_path, dt, warned):
+    """The reference's stability check (src/initial.f90:29-33): dt above min(h)^2 / 8
+    warns, else the limit is printed (16^3 on a box of 16: h = 1, limit 0.125)."""
+    _need_driver()
+    nml = tmp_path / "dt.nml"
+    _nml(nml, **dict(C1_16, dt_re=dt), ic_only=True, out_dir=str(tmp_path / "data"))
+    r = subprocess.run([DRV, str(nml)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert ("WARNING TIMESTEP IS PROBABLY TOO LARGE!" in r.stdout) == warned
+    if not warned:
+        line = next(ln for ln in r.stdout.splitlines() if "below upper-limit" in ln)
+        assert float(line.split()[-1]) == pytest.approx(0.125)
+
+
 def test_driver_errors_like_emergency_stop(tmp_path):
     _need_driver()
     nml = tmp_path / "bad.nml"

@@ -110,6 +110,9 @@ program tbec_run
   ! the reference's stability warning (src/initial.f90:29-33)
   if (dt_re > (1.0_dp / 8.0_dp) * min(Lx / Nx, Ly / Ny, Lz / Nz)**2) then
     write(*, '(a)') ' WARNING TIMESTEP IS PROBABLY TOO LARGE!'
+  else
+    write(*, '(a,es13.6)') ' dt is below upper-limit of', &
+      (1.0_dp / 8.0_dp) * min(Lx / Nx, Ly / Ny, Lz / Nz)**2
   end if
 
   allocate(xx(Nx), yy(Ny), zz(Nz), psi(Nz, Ny, Nx))
