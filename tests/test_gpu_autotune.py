@@ -75,6 +75,42 @@ def test_first_stage_tiling_changes_speed_not_results(prec):
             assert err < tol, (tile1, seg1, err)
 
 
+@pytest.mark.parametrize("setup", ["table_solid", "periodic_advective"])
+def test_first_stage_tiling_with_other_physics(setup):
+    """The first stage's own tiling with the trap read from a table, boundary faces that a
+    fill kernel refreshes before each stage, and the advective drift: the psi of the
+    single tiling (FP32 centred sums, 40 x 36 x 52)."""
+    shape = (40, 36, 52)
+    pr = initial.Problem(shape, (8.0, 7.0, 9.0), (1.0, 1.1, 0.9), 50.0, 0.0, 1e-3)
+    psi0 = initial.tf_profile(shape, pr.box, pr.omega, pr.g, norm=1.0)[0] * \
+        initial.vortex_line_factor(shape, pr.box, 0.3, 0.1, 1, 0.4)
+    print(f"\n[case] first-stage tiling {setup} {shape}", flush=True)
+
+    def run(tile1):
+        with tbec.Condensate.from_problem(pr, prec=32) as c:
+            if setup == "table_solid":
+                tab = c.potential(0.0)
+                c.set_physics("table")
+                c.set_potential_table(tab)
+                c.set_boundaries("solid")
+            else:
+                c.set_physics("harmonic", advective_vx=0.3)
+                c.set_boundaries("periodic", periodicz=True)
+            c.set_tuning(2, 0)
+            c.set_first_stage_tuning(tile1, 0)
+            c.upload(psi0)
+            c.step(3)
+            c.step(32)
+            return c.download()
+
+    ref = run(-1)
+    for tile1 in range(tbec.NUM_TILES):
+        got = run(tile1)
+        err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        print(f"first stage tile {tile1}: rel max {err:.2e}", flush=True)
+        assert err < 2e-6, (tile1, err)
+
+
 def test_first_stage_tuning_rejects_bad_arguments():
     pr = initial.Problem((16, 16, 16), (8.0, 8.0, 8.0), (1.0, 1.0, 1.0), 50.0, 0.0, 1e-3)
     with tbec.Condensate.from_problem(pr, prec=32) as c:
