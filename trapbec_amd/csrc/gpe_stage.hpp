@@ -110,6 +110,22 @@ template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T 
   e = (a - (s - bb)) + (b - bb);
 }
 
+// Explicit FMAs for the stage arithmetic: fmaT(a, b, c) = a*b + c rounded once, and
+// fma2(a, b, c, d) = a*b + c*d as fma(a, b, c*d).  Left to the compiler, the contraction
+// would not be the same in every unrolled copy of the plane loop: for a sum of two
+// products it may fuse either one, and a product that also occurs in another copy (the
+// x stencil's 270*q[x+1] of plane x is plane x+2's 270*q[x+1]) is computed once and then
+// not fused at all.  A plane's rounding would then depend on its position in the unrolled
+// loop, and so on the segment length.  Sums of two products and the stencil chains of
+// plane values go through these.
+template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+template <typename T> __device__ __forceinline__ T fma2(T a, T b, T c, T d) {
+  return fmaT(a, b, c * d);
+}
+
 template <typename T, int VZ> struct VecIO;
 template <> struct VecIO<float, 4> {   // 4 complex floats = 2 x 16 B
   static __device__ __forceinline__ void ld(const cplx<float> *p, cplx<float> *o) {
@@ -202,6 +218,9 @@ constexpr int kHaloLoadCpol = TBEC_HALO_LOAD_CPOL;
 #ifndef TBEC_UNIFORM_VMEM
 #define TBEC_UNIFORM_VMEM 1
 #endif
+#ifndef TBEC_AHEAD2
+#define TBEC_AHEAD2 1
+#endif
 // A buffer offset past every plane's record count (a plane is < 2^31 bytes): loads there
 // return zeros without a memory access, stores there are dropped.
 constexpr uint32_t kOffNone = 0x80000000u;
@@ -287,8 +306,12 @@ template <typename T, int VZ> struct BufIOR {
 template <typename T, int SUM>
 __device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
   if constexpr (SUM == SUM_REF) {
-    return (((((T(2) * m3 - T(27) * m2) + T(270) * m1) - T(490) * c) + T(270) * p1) -
-            T(27) * p2) + T(2) * p3;
+    T r = fma2(T(2), m3, T(-27), m2);
+    r = fmaT(T(270), m1, r);
+    r = fmaT(T(-490), c, r);
+    r = fmaT(T(270), p1, r);
+    r = fmaT(T(-27), p2, r);
+    return fmaT(T(2), p3, r);
   } else if constexpr (SUM == SUM_COMP) {
     T s1, e1, s2, e2, s3, e3;
     two_sum(m1, p1, s1, e1);
@@ -296,12 +319,12 @@ __device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
     two_sum(m3, p3, s3, e3);
     const T c2 = c + c;
     const T d1 = (s1 - c2) + e1, d2 = (s2 - c2) + e2, d3 = (s3 - c2) + e3;
-    return T(270) * d1 - T(27) * d2 + T(2) * d3;
+    return fma2(T(270), d1, T(-27), d2) + T(2) * d3;
   } else {
     const T d1 = (m1 - c) + (p1 - c);
     const T d2 = (m2 - c) + (p2 - c);
     const T d3 = (m3 - c) + (p3 - c);
-    return T(270) * d1 - T(27) * d2 + T(2) * d3;
+    return fma2(T(270), d1, T(-27), d2) + T(2) * d3;
   }
 }
 
@@ -310,13 +333,17 @@ __device__ __forceinline__ T d2sum(T m3, T m2, T m1, T c, T p1, T p2, T p3) {
 template <typename T>
 __device__ __forceinline__ T d2diff(T dm1, T dp1, T dm2, T dp2, T dm3, T dp3) {
   const T d1 = dm1 + dp1, d2 = dm2 + dp2, d3 = dm3 + dp3;
-  return T(270) * d1 - T(27) * d2 + T(2) * d3;
+  return fma2(T(270), d1, T(-27), d2) + T(2) * d3;
 }
 
 // 7-tap first derivative sum (without 1/(60 h)).
 template <typename T>
 __device__ __forceinline__ T d1sum(T m3, T m2, T m1, T p1, T p2, T p3) {
-  return ((((-m3 + T(9) * m2) - T(45) * m1) + T(45) * p1) - T(9) * p2) + p3;
+  T r = fmaT(T(9), m2, -m3);
+  r = fmaT(T(-45), m1, r);
+  r = fmaT(T(45), p1, r);
+  r = fmaT(T(-9), p2, r);
+  return r + p3;
 }
 
 // Workgroup-wide sum of 4 doubles; thread 0 writes them.
@@ -432,10 +459,21 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     }
   }
 
-  // Register queue of 8 plane slots: plane p lives in slot (p - xs + 3) & 7.  The loop
-  // is unrolled by 8 so every slot index is a compile-time constant (no register moves
-  // when the queue advances) and the LDS buffer / halo register parity is static too.
-  C q[8][VZ];
+  constexpr bool UNI = TBEC_UNIFORM_VMEM && sizeof(T) == 4 && FIX && !STO && KIND == ST_RK1;
+  // Prefetch distance of the own points: plane x+3+AH is loaded in the iteration of plane
+  // x (the halo of plane x+1 always one ahead).  AH2: two planes ahead, in the first RK
+  // stage of the real-time analytic-trap instances, which have the registers for it
+  // (512^3 FP32 on (64,32,2): 111 -> 120 VGPRs; the step 1.5351 -> 1.5236 ms, three
+  // alternating runs each on one box; three ahead, at 128 VGPRs, gains nothing more, and
+  // the halo two ahead as well spills).  The other instances would spill.
+  constexpr bool AH2 = TBEC_AHEAD2 && UNI && !IMAG && PHYS == PH_HARM;
+  constexpr int AH = AH2 ? 2 : 1;
+  // Register queue of QN plane slots: plane p lives in slot (p - xs + 3) % QN, and the
+  // halo of plane p in hb[(p - xs) & 1].  The loop is unrolled by QN so every slot index
+  // is a compile-time constant (no register moves when the queue advances; only the live
+  // slots take registers) and the LDS buffer / halo parity is static too (QN even).
+  constexpr int QN = AH2 ? 10 : 8;
+  C q[QN][VZ];
   C hb[2][VZ];
   // buffer resources for x-plane `x` of the three arrays: base = the plane's row -3,
   // column -4; record count = one plane (sx complex values, < 2^31 bytes: checked by
@@ -444,8 +482,8 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // plane (sx) per plane: a 64-bit scalar add, no multiply by the loop-varying x.
   const int pbytes = (int)(sx * (long long)sizeof(C));
   auto rs = [&](const C *plane) { return plane_rsrc(plane, pbytes); };
-  const C *p_own = pin + (long long)(xs + 4) * sx;     // own points of plane x+4
-  const C *p_halo = pin + (long long)(xs + 1) * sx;    // halo of plane x+1
+  const C *p_own = pin + (long long)(xs + 3 + AH) * sx;   // own points of plane x+3+AH
+  const C *p_halo = pin + (long long)(xs + 1) * sx;       // halo of plane x+1
   C *p_wnext = pw + (long long)(xs + 1) * sx;          // w (or psi_{s-1}) of plane x+1
   C *p_out = pout + (long long)xs * sx;                // outputs of plane x
   C *p_w = pw + (long long)xs * sx;
@@ -461,7 +499,6 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // rotating 525 -> 469 us (A/B on one box, kernel traces).  Stages 2 and 3, at the HBM
   // rate for their mix, are 1.8% slower with it (the extra address work of the dummy
   // halo loads), and the stored-w scheme and FP64 slower still, so they keep the branches.
-  constexpr bool UNI = TBEC_UNIFORM_VMEM && sizeof(T) == 4 && FIX && !STO && KIND == ST_RK1;
   const uint32_t ownb = own * (uint32_t)sizeof(C);
   const uint32_t hoffb = (UNI && !hl) ? kOffNone : hoff * (uint32_t)sizeof(C);
   auto scale_in = [&](C *o) {
@@ -479,7 +516,11 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     scale_in(o);
   };
 #pragma unroll
-  for (int k = 0; k < 7; ++k) ldo(rs(pin + (long long)(xs - 3 + k) * sx), ownb, q[k]);
+  for (int k = 0; k < 6 + AH; ++k) {
+    // planes past xe+2 are not used; more than one ahead they can lie past plane nx+3
+    const bool use = AH == 1 || xs - 3 + k <= xe + 2;
+    ldo(rs(pin + (long long)(use ? xs - 3 + k : 0) * sx), use ? ownb : kOffNone, q[k]);
+  }
   if (UNI || hl) ldh(rs(pin + (long long)xs * sx), hoffb, hb[0]);
   // Centred sums share first differences between neighbouring points (below), in the
   // instances with the stage kind fixed, analytic trap, or tabulated potential in real
@@ -502,10 +543,12 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const int vbytes = (int)(sx * (long long)sizeof(T));
   const T *p_v = VT ? reinterpret_cast<const T *>(a.vtab) + rb + (long long)xs * sx : nullptr;
   const uint32_t ownv = own * (uint32_t)sizeof(T);
-  // planes x .. x+VA of it in a ring of VS slots (plane x in slot R & (VS-1)).  In real
+  // planes x .. x+VA of it in a ring of VS slots (plane x in slot R % VS, QN % VS == 0:
+  // the instances that read it keep QN = 8).  In real
   // time the potential, a fourth stream, is prefetched two planes ahead (512^3 FP32: -1%
   // per step); the imaginary-time instances keep one plane (two would spill)
   constexpr int VA = IMAG ? 1 : 2, VS = VA == 1 ? 2 : 4;
+  static_assert(!VT || QN % VS == 0, "potential ring must divide the unroll");
   T vq[VS][VZ];
   if constexpr (VT) {
 #pragma unroll
@@ -548,10 +591,18 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   const uint32_t ownst = (yin && full) ? ownb : kOffNone;
 
   auto plane = [&](auto Rc, const int x) {
-    constexpr int R = decltype(Rc)::value;   // (x - xs) & 7
+    constexpr int R = decltype(Rc)::value;   // (x - xs) % QN
     constexpr int buf = R & 1;
-    // prefetch plane x+4 (own) and the halo of plane x+1
-    ldo(rs(p_own), ownb, q[(R + 7) & 7]);
+    // prefetch plane x+3+AH (own) and the halo of plane x+1.  Two ahead, the own plane
+    // x+5 is past the padded layout's last plane (nx+3) at x = nx-1, and past what the
+    // segment uses (plane xe+2) from x = xe-2: those loads are issued at an offset past the
+    // record count on a plane that exists (zeros, no memory access).
+    if constexpr (AH2) {
+      const bool use = x + 3 + AH <= xe + 2;
+      ldo(rs(use ? p_own : pin), use ? ownb : kOffNone, q[(R + 6 + AH) % QN]);
+    } else {
+      ldo(rs(p_own), ownb, q[(R + 6 + AH) % QN]);
+    }
     if (UNI || hl) ldh(rs(p_halo), hoffb, hb[buf ^ 1]);
     // (plane x+1 <= nx and plane x+VA <= nx+1 exist: 4 padding planes above the interior)
     if (stage_reads_w(stage) && (UNI || x + 1 < xe))
@@ -559,7 +610,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     if constexpr (VT) {
       if (UNI || x + VA < xe)
         BufIOR<T, VZ>::template ld<kPointwiseLoadCpol>(plane_rsrc(p_v + VA * sx, vbytes),
-                                                       ownv, vq[(R + VA) & (VS - 1)]);
+                                                       ownv, vq[(R + VA) % VS]);
     }
     const C *wv = wq[buf];
 
@@ -567,7 +618,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     {
       C *row = &lds[buf][3 + ty][4 + tz * VZ];
 #pragma unroll
-      for (int v = 0; v < VZ; ++v) row[v] = q[(R + 3) & 7][v];
+      for (int v = 0; v < VZ; ++v) row[v] = q[(R + 3) % QN][v];
       if (hl) {
         C *hr = &lds[buf][hrow][hcol];
 #pragma unroll
@@ -599,9 +650,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
       const C ym3 = lds[buf][ty][col], ym2 = lds[buf][ty + 1][col],
               ym1 = lds[buf][ty + 2][col], yp1 = lds[buf][ty + 4][col],
               yp2 = lds[buf][ty + 5][col], yp3 = lds[buf][ty + 6][col];
-      const C xm3 = q[R & 7][v], xm2 = q[(R + 1) & 7][v], xm1 = q[(R + 2) & 7][v];
-      const C f = q[(R + 3) & 7][v];
-      const C xp1 = q[(R + 4) & 7][v], xp2 = q[(R + 5) & 7][v], xp3 = q[(R + 6) & 7][v];
+      const C xm3 = q[R % QN][v], xm2 = q[(R + 1) % QN][v], xm1 = q[(R + 2) % QN][v];
+      const C f = q[(R + 3) % QN][v];
+      const C xp1 = q[(R + 4) % QN][v], xp2 = q[(R + 5) % QN][v], xp3 = q[(R + 6) % QN][v];
       const C zm3 = zr[v + 1], zm2 = zr[v + 2], zm1 = zr[v + 3], zp1 = zr[v + 5],
               zp2 = zr[v + 6], zp3 = zr[v + 7];
       T rr, ri;
@@ -621,7 +672,7 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         C zdp1 = {zp1.re - f.re, zp1.im - f.im};
         C zdm1 = {zm1.re - f.re, zm1.im - f.im};
         if (v + 1 < VZ) {
-          const C fn = q[(R + 3) & 7][(v + 1) % VZ];
+          const C fn = q[(R + 3) % QN][(v + 1) % VZ];
           zdp1 = {fn.re - f.re, fn.im - f.im};
         }
         if (v > 0) zdm1 = {-zd_prev.re, -zd_prev.im};
@@ -638,8 +689,8 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
                                 zp3.re - f.re);
         const T dzi = d2diff<T>(zdm1.im, zdp1.im, zm2.im - f.im, zp2.im - f.im, zm3.im - f.im,
                                 zp3.im - f.im);
-        rr = (dxr * k2x + dyr * k2y) + dzr * k2z;
-        ri = (dxi * k2x + dyi * k2y) + dzi * k2z;
+        rr = fma2(dxr, k2x, dyr, k2y) + dzr * k2z;
+        ri = fma2(dxi, k2x, dyi, k2y) + dzi * k2z;
       } else {
         const T dxr = d2sum<T, SUM>(xm3.re, xm2.re, xm1.re, f.re, xp1.re, xp2.re, xp3.re);
         const T dxi = d2sum<T, SUM>(xm3.im, xm2.im, xm1.im, f.im, xp1.im, xp2.im, xp3.im);
@@ -647,8 +698,8 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         const T dyi = d2sum<T, SUM>(ym3.im, ym2.im, ym1.im, f.im, yp1.im, yp2.im, yp3.im);
         const T dzr = d2sum<T, SUM>(zm3.re, zm2.re, zm1.re, f.re, zp1.re, zp2.re, zp3.re);
         const T dzi = d2sum<T, SUM>(zm3.im, zm2.im, zm1.im, f.im, zp1.im, zp2.im, zp3.im);
-        rr = (dxr * k2x + dyr * k2y) + dzr * k2z;
-        ri = (dxi * k2x + dyi * k2y) + dzi * k2z;
+        rr = fma2(dxr, k2x, dyr, k2y) + dzr * k2z;
+        ri = fma2(dxi, k2x, dyi, k2y) + dzi * k2z;
       }
       if (PHYS == PH_ROT || drift == 1) {
         if constexpr (sizeof(T) == 4) {
@@ -658,15 +709,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
           const T s1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im);
           const T s1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re);
           const T s1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im);
-          rr = (rr - omxk * s1yi) + omyk * s1xi;   // (0, om) * (x D1y - y D1x)
-          ri = (ri + omxk * s1yr) - omyk * s1xr;
+          rr = fmaT(omyk, s1xi, fmaT(-omxk, s1yi, rr));   // (0, om) * (x D1y - y D1x)
+          ri = fmaT(-omyk, s1xr, fmaT(omxk, s1yr, ri));
         } else {
           const T d1yr = d1sum<T>(ym3.re, ym2.re, ym1.re, yp1.re, yp2.re, yp3.re) * k1y;
           const T d1yi = d1sum<T>(ym3.im, ym2.im, ym1.im, yp1.im, yp2.im, yp3.im) * k1y;
           const T d1xr = d1sum<T>(xm3.re, xm2.re, xm1.re, xp1.re, xp2.re, xp3.re) * k1x;
           const T d1xi = d1sum<T>(xm3.im, xm2.im, xm1.im, xp1.im, xp2.im, xp3.im) * k1x;
-          const T pr = xc * d1yr - yc * d1xr;
-          const T pim = xc * d1yi - yc * d1xi;
+          const T pr = fma2(xc, d1yr, -yc, d1xr);
+          const T pim = fma2(xc, d1yi, -yc, d1xi);
           rr -= om * pim;   // (0, om) * (pr, pim)
           ri += om * pr;
         }
@@ -676,15 +727,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         rr -= vx2 * d1xi;  // (0, 2 vx) * (d1xr, d1xi)
         ri += vx2 * d1xr;
       }
-      const T n2 = f.re * f.re + f.im * f.im;
+      const T n2 = fma2(f.re, f.re, f.im, f.im);
       T pot;
-      if constexpr (VT) pot = vq[R & (VS - 1)][v];
+      if constexpr (VT) pot = vq[R % VS][v];
       else pot = potxy + zc2[v];
       const T fac = (c0 - gnl * n2) - pot;
       rr += fac * f.re;
       ri += fac * f.im;
 
-      const T ur = cre * rr - cim * ri, ui = cre * ri + cim * rr;
+      const T ur = fma2(cre, rr, -cim, ri), ui = fma2(cre, ri, cim, rr);
       if (stage == ST_RHS) {
         wo[v] = {rr, ri};
       } else if (stage == ST_RK1 || stage == ST_EULER) {
@@ -703,15 +754,15 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
         if (yin && zin[v]) {
           if (stage == ST_RK1 || stage == ST_RHS) {
             red[0] += (double)n2;
-            red[1] += (double)f.re * (double)rr + (double)f.im * (double)ri;
+            red[1] += fma2((double)f.re, (double)rr, (double)f.im, (double)ri);
             red[2] += (double)n2 * (double)n2;
-            red[3] += (double)rr * (double)rr + (double)ri * (double)ri;
+            red[3] += fma2((double)rr, (double)rr, (double)ri, (double)ri);
           } else if (stage == ST_RK3 || stage == ST_EULER) {
             if constexpr (PN32)
-              pn += out[v].re * out[v].re + out[v].im * out[v].im;
+              pn += fma2(out[v].re, out[v].re, out[v].im, out[v].im);
             else
-              red[0] += (double)out[v].re * (double)out[v].re +
-                        (double)out[v].im * (double)out[v].im;
+              red[0] += fma2((double)out[v].re, (double)out[v].re, (double)out[v].im,
+                             (double)out[v].im);
           }
         }
       }
@@ -765,7 +816,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   using I5 = std::integral_constant<int, 5>;
   using I6 = std::integral_constant<int, 6>;
   using I7 = std::integral_constant<int, 7>;
-  for (int x = xs; x < xe; x += 8) {
+  using I8 = std::integral_constant<int, 8>;
+  using I9 = std::integral_constant<int, 9>;
+  for (int x = xs; x < xe; x += QN) {
     plane(I0{}, x);
     if (x + 1 < xe) plane(I1{}, x + 1);
     if (x + 2 < xe) plane(I2{}, x + 2);
@@ -774,6 +827,10 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
     if (x + 5 < xe) plane(I5{}, x + 5);
     if (x + 6 < xe) plane(I6{}, x + 6);
     if (x + 7 < xe) plane(I7{}, x + 7);
+    if constexpr (QN == 10) {
+      if (x + 8 < xe) plane(I8{}, x + 8);
+      if (x + 9 < xe) plane(I9{}, x + 9);
+    }
   }
 }
 
