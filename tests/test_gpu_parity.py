@@ -145,7 +145,8 @@ def test_segment_length_does_not_change_bits(prec, omega):
     """The x-march is unrolled over a ring of plane slots (8 or 10), so with a segment
     length that is not a multiple of it a plane lands in a different unrolled copy than
     with another length.  Every copy must round the same way: segment lengths 8, 10, 13,
-    24 and the whole column give the same psi bit for bit (real time, eager steps and
+    24, the whole column, and 1 and 3 (segments shorter than the prefetch distance, the
+    last one a single plane) give the same psi bit for bit (real time, eager steps and
     graph replays; FP32 with the first RK stage on a tile of its own, as at 512^3)."""
     shape = (52, 40, 72)
     announce(f"segment lengths prec {prec} omega {omega} {shape}")
@@ -153,7 +154,8 @@ def test_segment_length_does_not_change_bits(prec, omega):
     harm = (4.0, 4.0, 9.0)
     tile = 2
     outs = []
-    for seg in (8, 10, 13, 24, 64):
+    segs = (8, 10, 13, 24, 64, 1, 3)
+    for seg in segs:
         with tbec.Condensate(shape, 8.0, harm, omega_z=omega, dt=1e-4, prec=prec,
                              tile=tile) as c:
             c.set_tuning(tile, seg)
@@ -163,7 +165,7 @@ def test_segment_length_does_not_change_bits(prec, omega):
             c.step(3)
             c.step(16)
             outs.append(c.download())
-    for seg, o in zip((10, 13, 24, 64), outs[1:]):
+    for seg, o in zip(segs[1:], outs[1:]):
         nd = int((o != outs[0]).sum())
         print(f"segment {seg} vs 8: {nd} values differ", flush=True)
         assert nd == 0, seg
