@@ -30,7 +30,8 @@
 // memory traffic.  The centre plane of the tile plus a 3-deep y halo and a 4-wide
 // (aligned) z halo is staged in LDS (double buffered, one barrier per plane) for the y
 // and z stencils.  Plane x+4 (own points) and plane x+1 (halo) are prefetched one
-// iteration ahead.  Consecutive workgroup ids are spread over the 8 XCDs by the
+// iteration ahead (own points two ahead, plane x+5, in the first RK stage of the
+// real-time analytic-trap instances).  Consecutive workgroup ids are spread over the 8 XCDs by the
 // dispatcher; the tile index is remapped so each XCD gets a contiguous block of tiles
 // (shared halos stay in that XCD's L2).
 //
