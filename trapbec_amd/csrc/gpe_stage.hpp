@@ -525,12 +525,16 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   }
   if (UNI || hl) ldh(rs(pin + (long long)xs * sx), hoffb, hb[0]);
   // Centred sums share first differences between neighbouring points (below), in the
-  // instances with the stage kind fixed, analytic trap, or tabulated potential in real
-  // time: the rotating, the imaginary-time tabulated and the run-time-kind instances
-  // spill with the extra registers.  xd1: x_i - x_{i-1} of the previous plane, per point
-  // (plane xs: slots 3 and 2).
-  constexpr bool SHARE_D1 =
-      SUM == SUM_CENTRED && FIX && (PHYS == PH_HARM || (PHYS == PH_GEN && !IMAG));
+  // instances with the stage kind fixed, and there in the analytic trap, in the tabulated
+  // potential in real time, and in the first RK stage of every physics: stages 2 and 3 of
+  // the rotating and of the imaginary-time tabulated instances spill 12-20 B with the
+  // extra registers, and the run-time-kind instances more.  The first stages of the
+  // rotating frame joined once the explicit FMAs had lowered their registers (real-time
+  // Omega = 0.7 step 1.6699 -> 1.6533 ms, 256^3 rotating imaginary time 233.0 -> 228.9
+  // us per step; real-time results bitwise unchanged).  xd1: x_i - x_{i-1} of the previous
+  // plane, per point (plane xs: slots 3 and 2).
+  constexpr bool SHARE_D1 = SUM == SUM_CENTRED && FIX &&
+      (PHYS == PH_HARM || (PHYS == PH_GEN && !IMAG) || KIND == ST_RK1);
   C xd1[VZ];
   if constexpr (SHARE_D1) {
 #pragma unroll
