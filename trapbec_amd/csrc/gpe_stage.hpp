@@ -467,7 +467,8 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // (512^3 FP32 on (64,32,2): 111 -> 120 VGPRs; the step 1.5351 -> 1.5236 ms, three
   // alternating runs each on one box; three ahead, at 128 VGPRs, gains nothing more, and
   // the halo two ahead as well spills).  The other FP32 instances would spill; the FP64
-  // first stage fits (98 -> 110 VGPRs) but its step was 0.5% slower (3.052 -> 3.067 ms).
+  // first stage fits (98 -> 110 VGPRs) but its step was 0.5% slower (3.052 -> 3.067 ms),
+  // and the rotating one (127 VGPRs) measured the same as one ahead (1.693 / 1.698 ms).
   constexpr bool AH2 = TBEC_AHEAD2 && UNI && !IMAG && PHYS == PH_HARM;
   constexpr int AH = AH2 ? 2 : 1;
   // Register queue of QN plane slots: plane p lives in slot (p - xs + 3) % QN, and the
