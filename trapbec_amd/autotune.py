@@ -18,7 +18,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 DEFAULT = {32: (2, 0), 64: (2, 0)}      # seg 0: libtbec picks it from the grid size
-SEGS = (8, 16, 32, 64, 128, 256)
+SEGS = (4, 8, 16, 32, 64, 128, 256)
 
 
 def _key(shape, prec, sum_mode=None, w_stored=False, rot=False):

@@ -67,11 +67,15 @@ long long tiles_per_plane(const tbec_params &p, int tile) {
 }
 
 // Segment length when none is given: the longest of 256, 128, ..., 8 planes that still
-// gives `want` workgroups (a few per CU on 256 CUs); 8 for grids too small for that.
+// gives `want` workgroups (a few per CU on 256 CUs).  Grids too small for that take 8, or
+// 4 where 8 would leave fewer than want/2 workgroups: there a shorter march's extra
+// prologue loads cost less than idle CUs (FP32, tile (64,16,2): 128^3 imaginary time
+// 54.6 -> 52.1 us per step, real time 40.3 -> 36.7; 64^3 42.9 -> 31.2 and 29.1 -> 20.4;
+// at 192^3 the 864 workgroups of segment 8 beat segment 4: 133.5 against 138.8 us).
 int auto_seg(long long tiles, int nx, long long want) {
   for (int seg = 256; seg > 8; seg >>= 1)
     if (tiles * ((nx + seg - 1) / seg) >= want) return seg;
-  return 8;
+  return 2 * tiles * ((nx + 7) / 8) < want ? 4 : 8;
 }
 
 hipError_t launch_stage(int prec, int tile, const StageArgs &a, int sum, int phys,

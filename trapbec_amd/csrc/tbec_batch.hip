@@ -130,15 +130,19 @@ int grid_blocks(long long n) {
 }
 
 // Automatic segment length for the whole ensemble: the items of all instances together
-// should number a few per CU; the largest instance's x extent bounds the search.
+// should number a few per CU, with the single-handle rule (auto_seg) applied to their sum:
+// the longest segment that gives kAutoSegBlocks items, else 8, or 4 where 8 gives fewer
+// than half of that.
 int batch_auto_seg(const tbec_batch *b, int tile) {
-  for (int seg = 256; seg > 8; seg >>= 1) {
-    long long items = 0;
+  auto items = [&](int seg) {
+    long long n = 0;
     for (int i = 0; i < b->n; ++i)
-      items += tiles_per_plane(b->p[i], tile) * ((b->p[i].nx + seg - 1) / seg);
-    if (items >= kAutoSegBlocks) return seg;
-  }
-  return 8;
+      n += tiles_per_plane(b->p[i], tile) * ((b->p[i].nx + seg - 1) / seg);
+    return n;
+  };
+  for (int seg = 256; seg > 8; seg >>= 1)
+    if (items(seg) >= kAutoSegBlocks) return seg;
+  return 2 * items(8) < kAutoSegBlocks ? 4 : 8;
 }
 
 // Build the device instance table for the current tile / segment choice.
