@@ -306,8 +306,10 @@ long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
 // Padded device layout of one condensate: interior z starts at a 128-byte boundary;
 // tiles of up to 128 in z and 64 in y may overrun the interior and read zeros; x has 3
-// ghost planes below and 4 above (the register queue prefetches plane x+4; the stage
-// kernel issues loads of plane x+5 only at an offset past the plane, when x+5 > nx+3).
+// ghost planes below and 4 above (the register queue prefetches plane x+4; where the stage
+// kernel prefetches plane x+5, the loads of planes past the segment's last used one,
+// x+5 > xe+2, which covers every plane past nx+3, go out at an offset past the record
+// count of an existing plane).
 void compute_layout(const tbec_params &p, Layout &L) {
   L.csize = p.prec == TBEC_F32 ? 8 : 16;
   L.hx = p.Lx / p.nx;
