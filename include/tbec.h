@@ -352,6 +352,12 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
 int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg);
 /* tbec_set_residual_gate for every instance of the batch (next solve). */
 int tbec_batch_set_residual_gate(tbec_batch *b, double rmax);
+/* on = 1: during a solve, once the unfinished instances' work items number under one per
+ * CU (256), they run on the shorter segment the automatic rule picks for them alone (the
+ * batch's own segment is restored when the solve returns).  Speeds up a long tail of a few
+ * slow instances; the FP64 reduction order of those instances then changes, so their
+ * results match a single-handle solve to rounding rather than bit for bit.  Default 0. */
+int tbec_batch_set_tail_segment(tbec_batch *b, int on);
 void *tbec_batch_stream(tbec_batch *b);
 
 /* ---- learned warm start ---------------------------------------------------------

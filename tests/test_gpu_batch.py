@@ -76,3 +76,52 @@ def test_batch_rejects_mixed_precision():
     probs = _problems()[:2]
     with pytest.raises(tbec.TbecError):
         tbec.Batch([p for p, _ in probs], prec=48)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_tail_segment(prec):
+    """With the tail option, once the unfinished instances' items fall under one per CU
+    they run on a shorter segment: the same instances converge to the same states (FP64
+    reduction order changes, so to rounding, not bit for bit); the batch's own segment is
+    back for the next solve, which then repeats a plain batch solve bit for bit."""
+    probs = _problems()
+    tile, seg = 2, 32            # long segments: the tail switches to the automatic 4
+    print(f"\n[case] batch tail segment prec={prec}", flush=True)
+
+    def solve(tail, twice=False):
+        with tbec.Batch([p for p, _ in probs], prec=prec) as b:
+            b.set_tuning(tile, seg)
+            runs = []
+            for k in range(2 if twice else 1):
+                b.set_tail_segment(tail and k == 0)
+                for i, (_, psi0) in enumerate(probs):
+                    b.upload(i, psi0)
+                res = b.imag_solve(2500, 1e-9, 1.0, 40)
+                runs.append((res, [b.download(i) for i in range(len(probs))]))
+            return runs
+
+    (plain_res, plain_psi), = solve(False)
+    (tail_res, tail_psi), (again_res, again_psi) = solve(True, twice=True)
+    assert len({int(r["steps"]) for r in plain_res}) > 1     # instances finish apart
+    rtol = 1e-11 if prec == 64 else 1e-6
+    for i in range(len(probs)):
+        assert tail_res[i]["converged"] == plain_res[i]["converged"], i
+        assert abs(int(tail_res[i]["steps"]) - int(plain_res[i]["steps"])) <= 80, i
+        assert tail_res[i]["energy"] == pytest.approx(plain_res[i]["energy"], rel=rtol), i
+        err = float(np.abs(tail_psi[i] - plain_psi[i]).max() / np.abs(plain_psi[i]).max())
+        print(f"instance {i}: steps {int(plain_res[i]['steps'])} / "
+              f"{int(tail_res[i]['steps'])}, psi rel max {err:.2e}", flush=True)
+        # FP32: the stop rule can fire a few checks apart on a state that still relaxes
+        # slowly, and psi moves by ~1e-4 in those steps; the energies agree to 1e-6
+        assert err < (1e-9 if prec == 64 else 1e-3), i
+        # the second solve of the same batch, tail option off: the plain solve again
+        assert int(again_res[i]["steps"]) == int(plain_res[i]["steps"]), i
+        assert np.array_equal(again_psi[i], plain_psi[i]), i
+
+
+def test_batch_tail_segment_rejects_bad_flag():
+    probs = _problems()[:2]
+    with tbec.Batch([p for p, _ in probs], prec=32) as b:
+        with pytest.raises(tbec.TbecError):
+            tbec._check(tbec.load().tbec_batch_set_tail_segment(b._h, 2))
+        b.set_tail_segment(False)

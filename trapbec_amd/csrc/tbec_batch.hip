@@ -39,6 +39,7 @@ struct tbec_batch {
   int total_items = 0;
   int active_items = 0;              // entries of d_items: the unfinished instances' items
   int active_done = 0;               // finished instances when d_items was last built
+  int tail_seg_on = 0;               // shorten the segment for a tail of few instances
   double rgate = 0.0;                // GP-residual gate of the stop rule (0: none)
   BatchInst *d_inst = nullptr;
   int *d_item_inst = nullptr;
@@ -143,6 +144,25 @@ int batch_auto_seg(const tbec_batch *b, int tile) {
   for (int seg = 256; seg > 8; seg >>= 1)
     if (items(seg) >= kAutoSegBlocks) return seg;
   return 2 * items(8) < kAutoSegBlocks ? 4 : 8;
+}
+
+// Segment for the unfinished instances alone, once their items number under one per CU
+// (kAutoSegBlocks / 4): the automatic rule applied to them, if shorter than the current
+// one; otherwise the current segment.  A long tail of a few slow instances (C4: one 96^3
+// instance for the last ~40% of the steps) then gets more workgroups.
+int tail_seg(const tbec_batch *b, const int *flags) {
+  auto items = [&](int seg) {
+    long long n = 0;
+    for (int i = 0; i < b->n; ++i)
+      if (flags[4 * i] == 0)
+        n += tiles_per_plane(b->p[i], b->tile) * ((b->p[i].nx + seg - 1) / seg);
+    return n;
+  };
+  if (items(b->seg) * 4 >= kAutoSegBlocks) return b->seg;
+  int seg = 2 * items(8) < kAutoSegBlocks ? 4 : 8;
+  for (int s = 256; s > 8; s >>= 1)
+    if (items(s) >= kAutoSegBlocks) { seg = s; break; }
+  return std::min(seg, b->seg);
 }
 
 // Build the device instance table for the current tile / segment choice.
@@ -417,6 +437,13 @@ int tbec_batch_set_tuning(tbec_batch *b, int tile, int seg) {
   return build_table(b);
 }
 
+int tbec_batch_set_tail_segment(tbec_batch *b, int on) {
+  if (!b) return set_error(TBEC_ERR_ARG, "null batch");
+  if (on != 0 && on != 1) return set_error(TBEC_ERR_ARG, "tail segment flag must be 0 or 1");
+  b->tail_seg_on = on;
+  return 0;
+}
+
 int tbec_batch_set_residual_gate(tbec_batch *b, double rmax) {
   if (!b) return set_error(TBEC_ERR_ARG, "null batch");
   if (!(rmax >= 0.0) || !std::isfinite(rmax))
@@ -513,6 +540,7 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
   if (!rc) rc = set_active_items(b, nullptr);   // every instance starts unfinished
   if (rc) return rc;
   const int p0 = b->parity;
+  const int seg0 = b->seg;
   long long chunk = check_every + (check_every & 1);
   long long done = 0;
   bool all = false;
@@ -540,11 +568,26 @@ int tbec_batch_imag_solve(tbec_batch *b, int64_t max_steps, double tol, double n
         last = now;
       }
     }
-    // finished instances leave the launches (their workgroups would only exit)
+    // finished instances leave the launches (their workgroups would only exit); with the
+    // tail option a tail of few unfinished instances also gets a shorter segment
+    // (restored after the solve, so that every solve starts with the batch's own tuning)
     if (!all && nconv > b->active_done) {
+      const int ts = b->tail_seg_on ? tail_seg(b, b->h_flags) : b->seg;
+      if (ts < b->seg) {
+        drop_graphs(b);
+        b->seg = ts;
+        rc = build_table(b);
+        if (rc) return rc;
+      }
       rc = set_active_items(b, b->h_flags);
       if (rc) return rc;
     }
+  }
+  if (b->seg != seg0) {
+    drop_graphs(b);
+    b->seg = seg0;
+    rc = build_table(b);
+    if (rc) return rc;
   }
   BCHK(hipMemcpyAsync(b->h_state, b->d_state, sizeof(double) * ST_COUNT * b->n,
                       hipMemcpyDeviceToHost, b->stream));

@@ -93,7 +93,8 @@ class TbecError(RuntimeError):
 EXPORTS = ["tbec_create", "tbec_destroy", "tbec_upload_psi", "tbec_download_psi",
            "tbec_upload_psi_device", "tbec_download_psi_device", "tbec_step", "tbec_rhs",
            "tbec_diagnose", "tbec_diagnose_noalloc", "tbec_imag_solve",
-           "tbec_set_residual_gate", "tbec_batch_set_residual_gate", "tbec_set_norm",
+           "tbec_set_residual_gate", "tbec_batch_set_residual_gate",
+           "tbec_batch_set_tail_segment", "tbec_set_norm",
            "tbec_sync", "tbec_stream", "tbec_stream_rates",
            "tbec_time_stages", "tbec_set_tuning", "tbec_set_first_stage_tuning",
            "tbec_last_error", "tbec_version",
@@ -136,6 +137,7 @@ def load(path: str = LIB_PATH):
                                     C.POINTER(C.c_double)]
     L.tbec_set_residual_gate.argtypes = [H, C.c_double]
     L.tbec_batch_set_residual_gate.argtypes = [H, C.c_double]
+    L.tbec_batch_set_tail_segment.argtypes = [H, C.c_int]
     L.tbec_set_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_first_stage_tuning.argtypes = [H, C.c_int, C.c_int]
     L.tbec_set_xcd_remap.argtypes = [H, C.c_int]
@@ -452,6 +454,11 @@ class Batch:
 
     def set_residual_gate(self, rmax):
         _check(load().tbec_batch_set_residual_gate(self._h, rmax))
+
+    def set_tail_segment(self, on: bool):
+        """Shorter segment for a tail of few unfinished instances during solves (results
+        then equal single-handle solves to rounding, not bit for bit)."""
+        _check(load().tbec_batch_set_tail_segment(self._h, int(bool(on))))
 
     def imag_solve(self, max_steps, tol, norm=1.0, check_every=50):
         res = (TbecSolveResult * self.n)()
