@@ -246,7 +246,7 @@ def run_c4_continuation(n_inst=48, tol=1e-10, gate=5e-4, coarse_gate=1e-4, facto
         bf.set_tuning(tb["tile"], tb["seg"])
         bc.set_tuning(tc["tile"], tc["seg"])
         # the last few unfinished instances run on shorter segments (C4: one 96^3 instance
-        # takes the last ~40% of the fine steps alone)
+        # runs the last ~70% of its 1.5 M fine steps alone)
         bf.set_tail_segment(True)
         bc.set_tail_segment(True)
         for i, (_, psi0) in enumerate(probs):
