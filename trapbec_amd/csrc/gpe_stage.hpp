@@ -469,6 +469,9 @@ __device__ __forceinline__ void march_tile(const StageArgs &a, const int lb, dou
   // the halo two ahead as well spills).  The other FP32 instances would spill; the FP64
   // first stage fits (98 -> 110 VGPRs) but its step was 0.5% slower (3.052 -> 3.067 ms),
   // and the rotating one (127 VGPRs) measured the same as one ahead (1.693 / 1.698 ms).
+  // Stages 2 and 3 two ahead fit only without their shared first differences (125
+  // VGPRs) and were slower: 1.5417 -> 1.5632 ms per step (without the shared
+  // differences alone, 1.5557).
   constexpr bool AH2 = TBEC_AHEAD2 && UNI && !IMAG && PHYS == PH_HARM;
   constexpr int AH = AH2 ? 2 : 1;
   // Register queue of QN plane slots: plane p lives in slot (p - xs + 3) % QN, and the
